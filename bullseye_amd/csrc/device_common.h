@@ -1,0 +1,107 @@
+// device_common.h -- shared device-side types for the gfx950 imaging kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bullseye_amd.h"
+
+namespace bam {
+
+// Compute paths of the gridding kernels (enum bullseye_compute_path without DEFAULT).
+enum Path : int { PATH_FP64 = 1, PATH_F32K = 2, PATH_F32 = 3 };
+
+// Correlation handling modes of the entry points (cpu_algorithm/wrapper.cpp:182-425).
+enum CorrMode : int { MODE_1 = 1, MODE_2 = 2, MODE_4 = 4, MODE_4J = 5, MODE_PSF = 6 };
+
+template <int MODE> struct ModeTraits { static constexpr int NCORR = 1; };
+template <> struct ModeTraits<MODE_2> { static constexpr int NCORR = 2; };
+template <> struct ModeTraits<MODE_4> { static constexpr int NCORR = 4; };
+template <> struct ModeTraits<MODE_4J> { static constexpr int NCORR = 4; };
+
+template <int PATH> struct PathTraits {  // FP32 paths
+    typedef float acc_t;
+    static constexpr bool KAHAN = (PATH == PATH_F32K);
+};
+template <> struct PathTraits<PATH_FP64> {
+    typedef double acc_t;
+    static constexpr bool KAHAN = false;
+};
+
+// Per-facet geometry computed on the host in FP64 (phase_transform_policies.h:110-134 and
+// baseline_transform_policies.h:117-143).
+struct FacetGeom {
+    double u_term, v_term;   // planar uvw transform: u -= w*u_term, v -= w*v_term
+    double dl, dm, dn;       // phase offset; rotor exp(i 2 pi (u dl + v dm + w dn))
+    double pad[3];
+};
+
+// One Romein stream: a lane group walks the rows [row_lo, row_hi) of baseline `baseline`
+// (in time order) for channels [chan_lo, chan_hi) of facet `facet`, serpentine over channels.
+// Streams are enumerated implicitly: index -> (facet, baseline, channel block, time segment).
+struct GridArgs {
+    // inputs (device pointers; element types are the ABI's)
+    const visibility_base_type *vis;           // complex [row][chan][npol]
+    const uvw_base_type *uvw;                  // [row][3]
+    const reference_wavelengths_base_type *wavelengths;  // [spw][chan]
+    const visibility_weights_base_type *weights;          // [row][chan][npol]
+    const uint8_t *flags;                      // [row][chan][npol]
+    const uint8_t *row_flags;                  // [row]
+    const uint32_t *field;                     // [row]
+    const uint32_t *spw;                       // [row]
+    const uint32_t *ant1, *ant2;               // [row]
+    const uint64_t *time_ids;                  // [row]
+    const visibility_base_type *jones;         // complex [t][ant][facet][spw][chan][4]
+    const uint8_t *enabled;                    // [spw*chan]
+    const uint64_t *cgi;                       // [spw*chan] grid slice per channel
+    const uint32_t *perm;                      // [row] rows sorted by baseline, time order kept
+    const uint32_t *bl_start;                  // [baseline_count + 1]
+    const FacetGeom *geom;                     // [facet]
+    const void *conv;                          // AA taps (acc_t) [csz]
+    const double *tapsum;                      // [OS+1] sum of the 2S+1 taps per fraction
+    // outputs
+    double *grid;                              // [facet][cube][ny][nx][pstride][2]
+    double *norm;                              // [facet*cube*ncorr]
+    unsigned long long *counters;              // [0] samples, [1] taps
+    int *error_flag;   // bit 0: singular Jones, 1: antenna id, 2: grid slice / spw / time id
+    // sizes
+    uint32_t nx, ny, S, W, OS, csz;
+    uint32_t chan, npol, spw_count, antenna_count, facets, cube, pstride, ntime;
+    uint32_t pol_idx[4];
+    uint32_t imaging_field;
+    uint32_t baselines, chan_per_stream, n_chan_blocks, n_segments;
+    uint64_t n_streams;
+    double u_scale, v_scale, off_x, off_y;
+    int phase, transform;
+};
+
+// ---------- small complex helpers --------------------------------------------------------
+template <typename T> struct cpx { T re, im; };
+
+template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T> __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T> __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return {a.re, -a.im}; }
+
+// Kahan-compensated running sum of one real component.  The true sum is s - c.
+template <typename T, bool KAHAN> struct acc1 {
+    T s, c;
+    __device__ __forceinline__ void zero() { s = 0; c = 0; }
+    __device__ __forceinline__ void add(T x) {
+        if (KAHAN) {
+            T y = x - c;
+            T t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += x;
+        }
+    }
+    __device__ __forceinline__ double value() const {
+        return KAHAN ? (double)s - (double)c : (double)s;
+    }
+};
+
+}  // namespace bam

@@ -1,0 +1,791 @@
+// wrapper.hip -- the C ABI of the gfx950 imaging library (include/bullseye_amd.h).
+//
+// One library context per process, like the reference (cpu_algorithm/wrapper.cpp:56-64,
+// gpu_algorithm/wrapper.cu:58-71).  Gridding accumulates on the device across chunks into an
+// FP64 grid that initLibrary seeds from the caller's output buffer; the final chunk converts it
+// back into the caller's buffer (gpu_algorithm/wrapper.cu:293-314).  Calls return an int status
+// and never throw across the ABI.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+
+namespace bam {
+hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
+int romein_max_width();
+size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
+hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
+                       uint32_t n_bl, uint32_t *keys, uint32_t *vals, uint32_t *keys_out,
+                       uint32_t *counts, uint32_t *perm, uint32_t *bl_start, void *ws,
+                       size_t ws_bytes, int *err, hipStream_t st);
+hipError_t grid_to_abi(const double *g, void *out, uint64_t slices_fc, uint32_t P,
+                       uint32_t pstride, uint64_t npix, hipStream_t st);
+hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P,
+                       uint32_t pstride, uint64_t npix, hipStream_t st);
+hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st);
+hipError_t normalize_device(void *out, const double *norm, uint64_t slices, uint64_t npix,
+                            hipStream_t st);
+hipError_t weight_uniformly_device(void *out, const void *sf, const uint64_t *cgi, uint32_t F,
+                                   uint32_t C, uint32_t P, uint32_t sfC, uint64_t npix,
+                                   hipStream_t st);
+hipError_t finalize_device(void *buf, uint32_t F, uint32_t n, uint32_t fstride, uint32_t nx,
+                           uint32_t ny, const double *tx, const double *ty, void *scratch,
+                           float *rtmp, bool place_real, hipStream_t st);
+
+// cpu_gpu_common/cu_common.h:63 -- a float constant in every precision build
+static const float ARCSEC_TO_RAD = (float)(M_PI / (180.0 * 3600.0));
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n ? n : 16);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct TimedCall {
+    hipEvent_t start, k0, k1, stop;
+    bool has_kernel;
+};
+
+struct State {
+    bool initialized = false;
+    bullseye_options opts{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    // geometry fixed at initLibrary
+    size_t F = 0, cube = 0, P = 0, pstride = 0, nx = 0, ny = 0, S = 0, OS = 0, csz = 0;
+    size_t sfC = 0, wplanes = 1;
+    bool psf = false;
+    int path = PATH_FP64;
+    // device buffers
+    DevBuf grid, psfgrid, norm, counters, errflag, conv, tapsum, geom, taper;
+    DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
+    DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
+        in_jones, in_wl, in_enabled, in_cgi;
+    DevBuf out_tmp, fft_scratch, fft_real;
+    std::vector<normalization_base_type> host_norm;
+    std::vector<double> taper_x, taper_y;
+    std::vector<FacetGeom> geo_host;  // kept alive for the asynchronous upload
+    // timing
+    std::vector<TimedCall> pending;
+    double grid_seconds = 0, inv_seconds = 0;
+    bullseye_stats stats{};
+    unsigned long long last_counters[2] = {0, 0};
+};
+static State g;
+
+static int fail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g.err = buf;
+    return 1;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+static bool dev_mode() { return g.opts.device_resident != 0; }
+
+// Copy `bytes` from the caller's array to a device staging buffer (host mode) or use it as
+// is (device-resident mode).
+static hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
+    if (!src) {
+        *out = nullptr;
+        return hipSuccess;
+    }
+    if (dev_mode()) {
+        *out = src;
+        return hipSuccess;
+    }
+    hipError_t e = b.ensure(bytes);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, g.stream);
+    *out = b.p;
+    return e;
+}
+
+// Image-plane response of the AA kernel along one axis, normalised to 1 at the centre
+// (the taper the finalize step divides out).
+static std::vector<double> taper_vector(const std::vector<double> &conv, size_t S, size_t OS,
+                                        size_t n) {
+    std::vector<double> t(n);
+    const double centre = (double)((S + 1) * OS);
+    double sum = 0;
+    for (double c : conv) sum += c;
+    for (size_t k = 0; k < n; ++k) {
+        const double l = (double)k - (double)(n / 2);
+        double acc = 0;
+        for (size_t i = 0; i < conv.size(); ++i)
+            acc += conv[i] * cos(2.0 * M_PI * ((double)i - centre) / (double)OS * l / (double)n);
+        t[k] = acc / sum;
+    }
+    return t;
+}
+
+static size_t grid_elems() { return g.F * g.cube * g.ny * g.nx * g.pstride * 2; }
+static size_t psf_elems() { return g.F * g.sfC * g.ny * g.nx * 2; }
+static size_t abi_grid_bytes() { return g.F * g.cube * g.P * g.ny * g.nx * 2 * sizeof(grid_base_type); }
+static size_t abi_psf_bytes() { return g.F * g.sfC * g.ny * g.nx * 2 * sizeof(grid_base_type); }
+
+static int check_device_errors() {
+    int flag = 0;
+    if (g.errflag.p) {
+        hipError_t e = hipMemcpy(&flag, g.errflag.p, sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("reading the device error flag: %s", hipGetErrorString(e));
+        if (flag) {
+            int zero = 0;
+            (void)hipMemcpy(g.errflag.p, &zero, sizeof(int), hipMemcpyHostToDevice);
+        }
+    }
+    if (flag & 1) return fail("JONES MATRIX SET CONTAINS SINGULAR MATRICIES");
+    if (flag & 2) return fail("antenna id out of range (antenna_count or baseline_count too small)");
+    if (flag & 4) return fail("channel grid index, spw id or timestamp id out of range");
+    return 0;
+}
+
+static int barrier_impl() {
+    if (!g.stream) return 0;
+    hipError_t e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) return fail("gridding stream failed: %s", hipGetErrorString(e));
+    for (auto &c : g.pending) {
+        float ms = 0, kms = 0;
+        (void)hipEventElapsedTime(&ms, c.start, c.stop);
+        g.grid_seconds += ms * 1e-3;
+        g.stats.copy_ms = 0;
+        if (c.has_kernel) {
+            (void)hipEventElapsedTime(&kms, c.k0, c.k1);
+            g.stats.kernel_ms = kms;
+            float pms = 0;
+            (void)hipEventElapsedTime(&pms, c.start, c.k0);
+            g.stats.prep_ms = pms;
+        }
+        (void)hipEventDestroy(c.start);
+        (void)hipEventDestroy(c.k0);
+        (void)hipEventDestroy(c.k1);
+        (void)hipEventDestroy(c.stop);
+    }
+    g.pending.clear();
+    if (g.counters.p) {
+        unsigned long long c[2];
+        e = hipMemcpy(c, g.counters.p, sizeof c, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("reading counters: %s", hipGetErrorString(e));
+        g.stats.samples_gridded = c[0] - g.last_counters[0];
+        g.stats.taps = c[1] - g.last_counters[1];
+        g.last_counters[0] = c[0];
+        g.last_counters[1] = c[1];
+    }
+    return check_device_errors();
+}
+
+// The device results of all chunks so far, copied into the caller's buffers (final chunk).
+static int publish_results(gridding_parameters *p, bool psf) {
+    if (!psf) {
+        void *dst = dev_mode() ? p->output_buffer : nullptr;
+        if (!dev_mode()) {
+            HIP_TRY(g.out_tmp.ensure(abi_grid_bytes()));
+            dst = g.out_tmp.p;
+        }
+        if (dst)
+            HIP_TRY(grid_to_abi(g.grid.as<double>(), dst, g.F * g.cube, (uint32_t)g.P,
+                                (uint32_t)g.pstride, g.nx * g.ny, g.stream));
+        HIP_TRY(hipMemcpyAsync(g.host_norm.data(), g.norm.p,
+                               g.host_norm.size() * sizeof(normalization_base_type),
+                               hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (!dev_mode() && p->output_buffer)
+            HIP_TRY(hipMemcpy(p->output_buffer, g.out_tmp.p, abi_grid_bytes(),
+                              hipMemcpyDeviceToHost));
+    } else {
+        void *dst = dev_mode() ? p->sampling_function_buffer : nullptr;
+        if (!dev_mode()) {
+            HIP_TRY(g.out_tmp.ensure(std::max(abi_psf_bytes(), g.out_tmp.bytes)));
+            dst = g.out_tmp.p;
+        }
+        if (dst)
+            HIP_TRY(grid_to_abi(g.psfgrid.as<double>(), dst, g.F * g.sfC, 1, 1, g.nx * g.ny,
+                                g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (!dev_mode() && p->sampling_function_buffer)
+            HIP_TRY(hipMemcpy(p->sampling_function_buffer, g.out_tmp.p, abi_psf_bytes(),
+                              hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+// Streams: channels walked together and time segments per baseline.
+static void choose_streams(const gridding_parameters *p, size_t F, size_t B, size_t rows, int W,
+                           bool same_slice, uint32_t &cps, uint32_t &nseg) {
+    const size_t C = p->channel_count;
+    cps = g.opts.channels_per_stream ? g.opts.channels_per_stream
+                                     : (uint32_t)(same_slice ? std::min<size_t>(C, 16) : 1);
+    cps = (uint32_t)std::max<size_t>(1, std::min<size_t>(cps, C));
+    const size_t nblocks = (C + cps - 1) / cps;
+    if (g.opts.rows_per_stream) {
+        const size_t per_bl = std::max<size_t>(1, rows / std::max<size_t>(1, B));
+        nseg = (uint32_t)std::max<size_t>(1, per_bl / g.opts.rows_per_stream);
+    } else {
+        // enough lane groups to fill 256 CUs x 4 SIMDs several waves deep
+        const int G = W * ((W + 7) / 8);
+        const size_t want_groups = (size_t)256 * 4 * 4 * (64 / std::max(G, 1));
+        const size_t have = F * B * nblocks;
+        nseg = 1;
+        const size_t per_bl = std::max<size_t>(1, rows / std::max<size_t>(1, B));
+        while (have * nseg < want_groups && per_bl / (nseg * 2) >= 16) nseg *= 2;
+    }
+}
+
+static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    const bool psf = mode == MODE_PSF;
+    if (p->wplanes > 1) return fail("w-projection (wplanes > 1) is served by the batched "
+                                    "w-projection entry, not this build's AA path");
+    const int W = (int)(2 * p->conv_support + 1);
+    if (W > romein_max_width())
+        return fail("conv_support %zu too large for the AA gridder (max full support %d)",
+                    (size_t)p->conv_support, romein_max_width());
+    if (p->conv_support != g.S || p->conv_oversample != g.OS)
+        return fail("conv_support/conv_oversample differ from initLibrary");
+    const int ncorr = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
+    if (!psf && (size_t)ncorr > g.P)
+        return fail("mode grids %d correlations but number_of_polarization_terms_being_gridded is %zu",
+                    ncorr, g.P);
+    if (psf && !g.psf) return fail("sampling function buffer was not set up at initLibrary");
+    if (!p->antenna_1_ids || !p->antenna_2_ids)
+        return fail("antenna_1_ids/antenna_2_ids are required (Romein gridding walks baselines)");
+    if (mode == MODE_4J && !p->jones_terms) return fail("jones_terms is NULL");
+    if (p->row_count > 0xffffffffull) return fail("row_count too large");
+    const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
+    const size_t nspw = std::max<size_t>(1, p->spw_count);
+    const size_t B = p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
+    if (mode == MODE_1) {
+        if (p->polarization_index >= npol) return fail("polarization_index out of range");
+    } else if (mode == MODE_2) {
+        if (p->polarization_index >= npol || p->second_polarization_index >= npol)
+            return fail("polarization indexes out of range");
+    } else if ((mode == MODE_4 || mode == MODE_4J) && npol < 4) {
+        return fail("4-correlation gridding needs 4 correlations per sample");
+    }
+
+    TimedCall tc{};
+    HIP_TRY(hipEventCreate(&tc.start));
+    HIP_TRY(hipEventCreate(&tc.k0));
+    HIP_TRY(hipEventCreate(&tc.k1));
+    HIP_TRY(hipEventCreate(&tc.stop));
+    HIP_TRY(hipEventRecord(tc.start, g.stream));
+
+    GridArgs a{};
+    const void *d;
+    HIP_TRY(stage(g.in_vis, psf ? nullptr : p->visibilities,
+                  rows * C * npol * 2 * sizeof(visibility_base_type), &d));
+    a.vis = (const visibility_base_type *)d;
+    HIP_TRY(stage(g.in_uvw, p->uvw_coords, rows * 3 * sizeof(uvw_base_type), &d));
+    a.uvw = (const uvw_base_type *)d;
+    HIP_TRY(stage(g.in_wl, p->reference_wavelengths,
+                  nspw * C * sizeof(reference_wavelengths_base_type), &d));
+    a.wavelengths = (const reference_wavelengths_base_type *)d;
+    HIP_TRY(stage(g.in_w, psf ? nullptr : p->visibility_weights,
+                  rows * C * npol * sizeof(visibility_weights_base_type), &d));
+    a.weights = (const visibility_weights_base_type *)d;
+    HIP_TRY(stage(g.in_flags, psf ? nullptr : p->flags, rows * C * npol, &d));
+    a.flags = (const uint8_t *)d;
+    HIP_TRY(stage(g.in_rowflags, p->flagged_rows, rows, &d));
+    a.row_flags = (const uint8_t *)d;
+    HIP_TRY(stage(g.in_field, p->field_array, rows * 4, &d));
+    a.field = (const uint32_t *)d;
+    HIP_TRY(stage(g.in_spw, p->spw_index_array, rows * 4, &d));
+    a.spw = (const uint32_t *)d;
+    HIP_TRY(stage(g.in_a1, p->antenna_1_ids, rows * 4, &d));
+    a.ant1 = (const uint32_t *)d;
+    HIP_TRY(stage(g.in_a2, p->antenna_2_ids, rows * 4, &d));
+    a.ant2 = (const uint32_t *)d;
+    HIP_TRY(stage(g.in_enabled, p->enabled_channels, nspw * C, &d));
+    a.enabled = (const uint8_t *)d;
+    HIP_TRY(stage(g.in_cgi, psf ? p->sampling_function_channel_grid_indicies
+                                : p->channel_grid_indicies,
+                  nspw * C * sizeof(size_t), &d));
+    a.cgi = (const uint64_t *)d;
+    if (!a.uvw || !a.wavelengths || !a.row_flags || !a.field || !a.spw || !a.enabled || !a.cgi ||
+        (!psf && (!a.vis || !a.weights || !a.flags)))
+        return fail("a required array pointer is NULL");
+    if (mode == MODE_4J) {
+        HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
+        a.time_ids = (const uint64_t *)d;
+        const size_t nj = p->no_timestamps_read * p->antenna_count * g.F * nspw * C;
+        HIP_TRY(stage(g.in_jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d));
+        a.jones = (const visibility_base_type *)d;
+        if (!a.time_ids) return fail("timestamp_ids is NULL");
+    }
+
+    // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre)
+    HIP_TRY(hipStreamSynchronize(g.stream));  // the previous call's upload of geo_host is done
+    std::vector<FacetGeom> &geo = g.geo_host;
+    geo.assign(g.F, FacetGeom{});
+    for (size_t f = 0; f < g.F; ++f) {
+        FacetGeom &fg = geo[f];
+        memset(&fg, 0, sizeof fg);
+        if (!facet) continue;
+        const double ra0 = p->phase_centre_ra, dec0 = p->phase_centre_dec;
+        const double ra = p->facet_centres[2 * f], dec = p->facet_centres[2 * f + 1];
+        const double d_ra = (ra - ra0) * (double)ARCSEC_TO_RAD;
+        const double n_dec = dec * (double)ARCSEC_TO_RAD, o_dec = dec0 * (double)ARCSEC_TO_RAD;
+        // baseline_transform_policies.h:120-143
+        const double li0 = cos(n_dec) * sin(d_ra);
+        const double mi0 = sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra);
+        const double ni0 = sin(n_dec) * sin(o_dec) + cos(n_dec) * cos(o_dec) * cos(d_ra);
+        fg.u_term = li0 / ni0;
+        fg.v_term = mi0 / ni0;
+        if (phase) {  // phase_transform_policies.h:123-133
+            fg.dl = -cos(n_dec) * sin(d_ra);
+            fg.dm = -(sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra));
+            fg.dn = 1 - sqrt(1 - fg.dl * fg.dl - fg.dm * fg.dm);
+        }
+    }
+    HIP_TRY(g.geom.ensure(g.F * sizeof(FacetGeom)));
+    HIP_TRY(hipMemcpyAsync(g.geom.p, geo.data(), g.F * sizeof(FacetGeom), hipMemcpyHostToDevice,
+                           g.stream));
+
+    // ingest: stable sort of rows by baseline
+    HIP_TRY(g.perm.ensure(rows * 4));
+    HIP_TRY(g.keys.ensure(rows * 4));
+    HIP_TRY(g.vals.ensure(rows * 4));
+    HIP_TRY(g.keys_out.ensure(rows * 4));
+    HIP_TRY(g.counts.ensure((B + 1) * 4));
+    HIP_TRY(g.blstart.ensure((B + 1) * 4));
+    const size_t wsb = ingest_workspace_bytes((uint32_t)rows, (uint32_t)B);
+    HIP_TRY(g.ws.ensure(wsb));
+    HIP_TRY(ingest_sort(a.ant1, a.ant2, (uint32_t)rows, (uint32_t)p->antenna_count, (uint32_t)B,
+                        g.keys.as<uint32_t>(), g.vals.as<uint32_t>(), g.keys_out.as<uint32_t>(),
+                        g.counts.as<uint32_t>(), g.perm.as<uint32_t>(), g.blstart.as<uint32_t>(),
+                        g.ws.p, g.ws.bytes, g.errflag.as<int>(), g.stream));
+
+    // do all enabled channels map to one grid slice? (then channels can share a stream)
+    bool same_slice = true;
+    if (!dev_mode()) {
+        const size_t *cg = psf ? p->sampling_function_channel_grid_indicies : p->channel_grid_indicies;
+        long first = -1;
+        for (size_t i = 0; i < nspw * C; ++i) {
+            if (!p->enabled_channels[i]) continue;
+            if (first < 0) first = (long)cg[i];
+            else if ((long)cg[i] != first) same_slice = false;
+        }
+    } else {
+        same_slice = g.cube == 1 && (!psf || g.sfC == 1);
+    }
+    uint32_t cps, nseg;
+    choose_streams(p, g.F, B, rows, W, same_slice, cps, nseg);
+
+    a.perm = g.perm.as<uint32_t>();
+    a.bl_start = g.blstart.as<uint32_t>();
+    a.geom = g.geom.as<FacetGeom>();
+    a.conv = g.conv.p;
+    a.tapsum = g.tapsum.as<double>();
+    a.grid = psf ? g.psfgrid.as<double>() : g.grid.as<double>();
+    a.norm = g.norm.as<double>();
+    a.counters = g.counters.as<unsigned long long>();
+    a.error_flag = g.errflag.as<int>();
+    a.nx = (uint32_t)g.nx;
+    a.ny = (uint32_t)g.ny;
+    a.S = (uint32_t)g.S;
+    a.W = (uint32_t)W;
+    a.OS = (uint32_t)g.OS;
+    a.csz = (uint32_t)g.csz;
+    a.chan = (uint32_t)C;
+    a.npol = (uint32_t)npol;
+    a.spw_count = (uint32_t)nspw;
+    a.antenna_count = (uint32_t)p->antenna_count;
+    a.ntime = (uint32_t)p->no_timestamps_read;
+    a.facets = (uint32_t)g.F;
+    a.cube = (uint32_t)(psf ? g.sfC : g.cube);
+    a.pstride = (uint32_t)(psf ? 1 : g.pstride);
+    for (int k = 0; k < 4; ++k) a.pol_idx[k] = (uint32_t)k;
+    if (mode == MODE_1) a.pol_idx[0] = (uint32_t)p->polarization_index;
+    if (mode == MODE_2) {
+        a.pol_idx[0] = (uint32_t)p->polarization_index;
+        a.pol_idx[1] = (uint32_t)p->second_polarization_index;
+    }
+    a.imaging_field = p->imaging_field;
+    a.baselines = (uint32_t)B;
+    a.chan_per_stream = cps;
+    a.n_chan_blocks = (uint32_t)((C + cps - 1) / cps);
+    a.n_segments = nseg;
+    a.n_streams = (uint64_t)B * a.n_chan_blocks * nseg * g.F;
+    // templated_gridder.h:23-27 (u_scale uses the float ARCSEC_TO_RAD constant)
+    a.u_scale = (double)((double)p->nx * (double)p->cell_size_x * (double)ARCSEC_TO_RAD);
+    a.v_scale = -(double)((double)p->ny * (double)p->cell_size_y * (double)ARCSEC_TO_RAD);
+    a.off_x = (double)(p->nx / 2 - p->conv_support);
+    a.off_y = (double)(p->ny / 2 - p->conv_support);
+    a.phase = phase ? 1 : 0;
+    a.transform = facet ? 1 : 0;
+
+    HIP_TRY(hipEventRecord(tc.k0, g.stream));
+    HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
+    HIP_TRY(hipEventRecord(tc.k1, g.stream));
+    tc.has_kernel = true;
+    HIP_TRY(hipEventRecord(tc.stop, g.stream));
+    g.pending.push_back(tc);
+    if (!dev_mode()) {
+        // the caller may reuse its host arrays after gridding_barrier; staging copies are
+        // complete once the stream reaches the kernel
+    }
+    if (p->is_final_data_chunk) {
+        if (barrier_impl()) return 1;
+        if (publish_results(p, psf)) return 1;
+    }
+    return 0;
+}
+
+}  // namespace bam
+
+using namespace bam;
+
+extern "C" {
+
+const char *bullseye_last_error(void) { return g.err.c_str(); }
+
+int bullseye_set_options(const bullseye_options *o) {
+    if (!o) return fail("NULL options");
+    if (o->struct_size != sizeof(bullseye_options))
+        return fail("bullseye_options.struct_size %u != %zu", o->struct_size, sizeof(bullseye_options));
+    if (o->compute_path > BULLSEYE_PATH_FP32) return fail("bad compute_path %u", o->compute_path);
+    g.opts = *o;
+    if (g.initialized && o->compute_path)
+        g.path = (int)o->compute_path;
+    return 0;
+}
+
+int bullseye_get_options(bullseye_options *o) {
+    if (!o) return fail("NULL options");
+    *o = g.opts;
+    o->struct_size = sizeof(bullseye_options);
+    return 0;
+}
+
+int bullseye_get_stats(bullseye_stats *s) {
+    if (!s) return fail("NULL stats");
+    *s = g.stats;
+    return 0;
+}
+
+int bullseye_reset_grids(void) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (barrier_impl()) return 1;
+    HIP_TRY(fill_f64(g.grid.as<double>(), grid_elems(), 0.0, g.stream));
+    if (g.psf) HIP_TRY(fill_f64(g.psfgrid.as<double>(), psf_elems(), 0.0, g.stream));
+    HIP_TRY(fill_f64(g.norm.as<double>(), g.host_norm.size(), 0.0, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    std::fill(g.host_norm.begin(), g.host_norm.end(), 0.0);
+    return 0;
+}
+
+double get_gridding_walltime(void) { return g.grid_seconds; }
+double get_inversion_walltime(void) { return g.inv_seconds; }
+
+void gridding_barrier(void) { (void)barrier_impl(); }
+
+int initLibrary(gridding_parameters *p) {
+    if (g.initialized) return 0;
+    if (!p) return fail("NULL params");
+    g.err.clear();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device");
+    HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    g.F = std::max<size_t>(1, p->num_facet_centres);
+    g.cube = std::max<size_t>(1, p->cube_channel_dim_size);
+    g.P = std::max<size_t>(1, p->number_of_polarization_terms_being_gridded);
+    g.pstride = g.P;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.S = p->conv_support;
+    g.OS = p->conv_oversample;
+    g.wplanes = std::max<size_t>(1, p->wplanes);
+    g.psf = p->should_grid_sampling_function;
+    g.sfC = g.psf ? std::max<size_t>(1, p->sampling_function_channel_count) : 0;
+    g.csz = (2 * g.S + 3) + (2 * g.S + 2) * (g.OS - 1);
+    g.path = g.opts.compute_path ? (int)g.opts.compute_path
+#if defined(BULLSEYE_DOUBLE)
+                                 : PATH_FP64;
+#else
+                                 : PATH_F32K;
+#endif
+    if (g.nx == 0 || g.ny == 0 || g.OS == 0) return fail("nx, ny and conv_oversample must be > 0");
+    if (2 * g.S + 3 >= g.nx || 2 * g.S + 3 >= g.ny) return fail("conv_support too large for the grid");
+    if (!p->conv) return fail("conv is NULL");
+    HIP_TRY(g.grid.ensure(grid_elems() * sizeof(double)));
+    HIP_TRY(g.norm.ensure(g.F * g.cube * g.P * sizeof(double)));
+    HIP_TRY(g.counters.ensure(2 * sizeof(unsigned long long)));
+    HIP_TRY(g.errflag.ensure(sizeof(int)));
+    HIP_TRY(hipMemset(g.counters.p, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(g.errflag.p, 0, sizeof(int)));
+    g.last_counters[0] = g.last_counters[1] = 0;
+    g.host_norm.assign(g.F * g.cube * g.P, 0.0);
+    p->normalization_terms = g.host_norm.data();
+    HIP_TRY(fill_f64(g.norm.as<double>(), g.host_norm.size(), 0.0, g.stream));
+    // seed the accumulation grid from the caller's buffer (gpu_algorithm/wrapper.cu:165-168)
+    const void *src = nullptr;
+    if (p->output_buffer) {
+        if (dev_mode()) {
+            src = p->output_buffer;
+        } else {
+            HIP_TRY(g.out_tmp.ensure(abi_grid_bytes()));
+            HIP_TRY(hipMemcpy(g.out_tmp.p, p->output_buffer, abi_grid_bytes(), hipMemcpyHostToDevice));
+            src = g.out_tmp.p;
+        }
+    }
+    HIP_TRY(abi_to_grid(src, g.grid.as<double>(), g.F * g.cube, (uint32_t)g.P, (uint32_t)g.pstride,
+                        g.nx * g.ny, g.stream));
+    if (g.psf) {
+        HIP_TRY(g.psfgrid.ensure(psf_elems() * sizeof(double)));
+        const void *ps = nullptr;
+        if (p->sampling_function_buffer) {
+            if (dev_mode()) {
+                ps = p->sampling_function_buffer;
+            } else {
+                HIP_TRY(hipStreamSynchronize(g.stream));
+                HIP_TRY(g.out_tmp.ensure(std::max(abi_psf_bytes(), g.out_tmp.bytes)));
+                HIP_TRY(hipMemcpy(g.out_tmp.p, p->sampling_function_buffer, abi_psf_bytes(),
+                                  hipMemcpyHostToDevice));
+                ps = g.out_tmp.p;
+            }
+        }
+        HIP_TRY(abi_to_grid(ps, g.psfgrid.as<double>(), g.F * g.sfC, 1, 1, g.nx * g.ny, g.stream));
+    }
+    // kernel taps: FP64 and FP32 copies (the compute path picks one), tap sums per fraction
+    if (g.wplanes <= 1) {
+        std::vector<convolution_base_type> ch(g.csz);
+        if (dev_mode())
+            HIP_TRY(hipMemcpy(ch.data(), p->conv, g.csz * sizeof(convolution_base_type),
+                              hipMemcpyDeviceToHost));
+        else
+            memcpy(ch.data(), p->conv, g.csz * sizeof(convolution_base_type));
+        std::vector<double> cd(ch.begin(), ch.end());
+        std::vector<double> ts(g.OS + 1, 0.0);
+        for (size_t f = 0; f <= g.OS; ++f)
+            for (size_t k = 0; k < 2 * g.S + 1; ++k) ts[f] += cd[f + k * g.OS];
+        const bool fp64 = g.path == PATH_FP64;
+        HIP_TRY(g.conv.ensure(g.csz * 8));
+        if (fp64) {
+            HIP_TRY(hipMemcpy(g.conv.p, cd.data(), g.csz * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> cf(cd.begin(), cd.end());
+            HIP_TRY(hipMemcpy(g.conv.p, cf.data(), g.csz * 4, hipMemcpyHostToDevice));
+        }
+        HIP_TRY(g.tapsum.ensure((g.OS + 1) * 8));
+        HIP_TRY(hipMemcpy(g.tapsum.p, ts.data(), (g.OS + 1) * 8, hipMemcpyHostToDevice));
+        g.taper_x = taper_vector(cd, g.S, g.OS, g.nx);
+        g.taper_y = taper_vector(cd, g.S, g.OS, g.ny);
+        HIP_TRY(g.taper.ensure((g.nx + g.ny) * 8));
+        HIP_TRY(hipMemcpy(g.taper.p, g.taper_x.data(), g.nx * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.taper.as<double>() + g.nx, g.taper_y.data(), g.ny * 8,
+                          hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    g.grid_seconds = g.inv_seconds = 0;
+    g.stats = bullseye_stats{};
+    g.initialized = true;
+    return 0;
+}
+
+int releaseLibrary(void) {
+    if (!g.initialized) return 0;
+    (void)barrier_impl();
+    DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv, &g.tapsum,
+                      &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
+                      &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
+                      &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
+                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
+                      &g.fft_scratch, &g.fft_real};
+    for (DevBuf *b : bufs) b->release();
+    (void)hipStreamDestroy(g.stream);
+    g.stream = nullptr;
+    g.initialized = false;
+    return 0;
+}
+
+// cpu_algorithm/wrapper.cpp:116-135
+int weight_uniformly(gridding_parameters *p) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (barrier_impl()) return 1;
+    if (!p->sampling_function_buffer || !p->output_buffer) return fail("buffers are NULL");
+    const size_t F = g.F, C = g.cube, P = g.P, sfC = p->sampling_function_channel_count;
+    const size_t npix = p->nx * p->ny;
+    if (dev_mode()) {
+        HIP_TRY(weight_uniformly_device(p->output_buffer, p->sampling_function_buffer,
+                                        (const uint64_t *)p->channel_grid_indicies, (uint32_t)F,
+                                        (uint32_t)C, (uint32_t)P, (uint32_t)sfC, npix, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        return 0;
+    }
+    grid_base_type *out = (grid_base_type *)p->output_buffer;
+    const grid_base_type *sf = (const grid_base_type *)p->sampling_function_buffer;
+    for (size_t f = 0; f < F; ++f)
+        for (size_t gi = 0; gi < C; ++gi)
+            for (size_t pix = 0; pix < npix; ++pix) {
+                grid_base_type count = 0.0000001f;
+                for (size_t c = 0; c < sfC; ++c)
+                    count += (int)(p->channel_grid_indicies[c] == gi) * sf[((f * sfC + c) * npix + pix) * 2];
+                count = 1 / count;
+                for (size_t corr = 0; corr < P; ++corr) {
+                    grid_base_type *cell = out + (((f * C + gi) * P + corr) * npix + pix) * 2;
+                    cell[0] *= count;
+                    cell[1] *= count;
+                }
+            }
+    return 0;
+}
+
+// cpu_algorithm/wrapper.cpp:136-163
+int normalize(gridding_parameters *p) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (barrier_impl()) return 1;
+    const size_t slices = g.F * g.cube * g.P, npix = p->nx * p->ny;
+    if (dev_mode()) {
+        HIP_TRY(normalize_device(p->output_buffer, g.norm.as<double>(), slices, npix, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        return 0;
+    }
+    grid_base_type *out = (grid_base_type *)p->output_buffer;
+    const normalization_base_type *nt = p->normalization_terms ? p->normalization_terms : g.host_norm.data();
+    for (size_t s = 0; s < slices; ++s) {
+        const grid_base_type d = (grid_base_type)nt[s];
+        grid_base_type *sl = out + s * npix * 2;
+        for (size_t i = 0; i < npix; ++i) {
+            sl[2 * i] /= d;
+            sl[2 * i + 1] /= d;
+        }
+    }
+    return 0;
+}
+
+static int finalize_impl(gridding_parameters *p, bool psf) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (barrier_impl()) return 1;
+    void *buf = psf ? p->sampling_function_buffer : p->output_buffer;
+    if (!buf) return fail("buffer is NULL");
+    const uint32_t F = (uint32_t)g.F;
+    const uint32_t n = (uint32_t)(psf ? p->sampling_function_channel_count : p->cube_channel_dim_size);
+    const uint32_t fstride = (uint32_t)(psf ? p->sampling_function_channel_count
+                                            : p->cube_channel_dim_size * g.P);
+    const size_t npix = p->nx * p->ny;
+    const size_t total_bytes = (size_t)F * fstride * npix * 2 * sizeof(grid_base_type);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    void *dbuf = buf;
+    if (!dev_mode()) {
+        HIP_TRY(g.out_tmp.ensure(std::max(total_bytes, g.out_tmp.bytes)));
+        HIP_TRY(hipMemcpyAsync(g.out_tmp.p, buf, total_bytes, hipMemcpyHostToDevice, g.stream));
+        dbuf = g.out_tmp.p;
+    }
+    HIP_TRY(g.fft_scratch.ensure((size_t)F * n * npix * 2 * sizeof(grid_base_type)));
+    HIP_TRY(g.fft_real.ensure((size_t)F * n * npix * sizeof(float)));
+    const bool taper = g.opts.apply_taper_correction && !psf && g.taper.p;
+    const double *tx = taper ? g.taper.as<double>() : nullptr;
+    const double *ty = taper ? g.taper.as<double>() + g.nx : nullptr;
+    HIP_TRY(finalize_device(dbuf, F, n, fstride, (uint32_t)p->nx, (uint32_t)p->ny, tx, ty,
+                            g.fft_scratch.p, g.fft_real.as<float>(), dev_mode(), g.stream));
+    HIP_TRY(hipEventRecord(e1, g.stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    g.inv_seconds += ms * 1e-3;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!dev_mode()) {
+        HIP_TRY(hipMemcpy(buf, g.out_tmp.p, total_bytes, hipMemcpyDeviceToHost));
+        // fft_and_repacking_routines.cpp:106-116: real parts as float32, in place
+        const grid_base_type *gt = (const grid_base_type *)buf;
+        float *fl = (float *)buf;
+        const size_t offset = (size_t)fstride * npix;
+        for (size_t f = 0; f < F; ++f)
+            for (size_t i = offset * f; i < offset * f + (size_t)n * npix; ++i) fl[i] = (float)gt[i * 2];
+    }
+    return 0;
+}
+
+int finalize(gridding_parameters *p) { return finalize_impl(p, false); }
+int finalize_psf(gridding_parameters *p) { return finalize_impl(p, true); }
+
+// gpu_algorithm/wrapper.cu:358-454: the reference's host-side reorder of the row arrays by
+// baseline.  The gridder does not need it (it sorts on the device); it is kept so callers that
+// invoke it see the same side effects.
+int repack_input_data(gridding_parameters *p) {
+    if (dev_mode()) return fail("repack_input_data works on host arrays; device-resident rows "
+                                "need no repacking");
+    const size_t rows = p->row_count, A = p->antenna_count, B = p->baseline_count;
+    const size_t per = p->channel_count * p->number_of_polarization_terms;
+    if (!p->baseline_starting_indexes) return fail("baseline_starting_indexes is NULL");
+    auto bidx = [A](long a1, long a2) {
+        long s = std::min(a1, a2);
+        return (size_t)((s * (-s + (2 * (long)A + 1))) / 2 + std::labs(a1 - a2));
+    };
+    size_t *st = p->baseline_starting_indexes;
+    memset(st, 0, sizeof(size_t) * (B + 1));
+    for (size_t r = 0; r < rows; ++r) {
+        size_t b = bidx(p->antenna_1_ids[r], p->antenna_2_ids[r]);
+        if (b >= B) return fail("baseline index out of range");
+        ++st[b + 1];
+    }
+    for (size_t b = 0; b < B; ++b) st[b + 1] += st[b];
+    std::vector<size_t> cur(B, 0), dst(rows);
+    for (size_t r = 0; r < rows; ++r) {
+        size_t b = bidx(p->antenna_1_ids[r], p->antenna_2_ids[r]);
+        dst[r] = st[b] + cur[b]++;
+    }
+    auto permute = [&](void *base, size_t elem) {
+        if (!base) return;
+        std::vector<unsigned char> tmp(rows * elem);
+        for (size_t r = 0; r < rows; ++r)
+            memcpy(&tmp[dst[r] * elem], (unsigned char *)base + r * elem, elem);
+        memcpy(base, tmp.data(), rows * elem);
+    };
+    permute(p->uvw_coords, 3 * sizeof(uvw_base_type));
+    permute(p->visibilities, per * 2 * sizeof(visibility_base_type));
+    permute(p->visibility_weights, per * sizeof(visibility_weights_base_type));
+    permute(p->flags, per);
+    permute(p->flagged_rows, 1);
+    permute(p->spw_index_array, 4);
+    permute(p->antenna_1_ids, 4);
+    permute(p->antenna_2_ids, 4);
+    permute(p->field_array, 4);
+    if (p->should_invert_jones_terms) permute(p->timestamp_ids, sizeof(size_t));
+    return 0;
+}
+
+int grid_single_pol(gridding_parameters *p) { return grid_call(p, MODE_1, false, false); }
+int facet_single_pol(gridding_parameters *p) { return grid_call(p, MODE_1, true, true); }
+int grid_duel_pol(gridding_parameters *p) { return grid_call(p, MODE_2, false, false); }
+int facet_duel_pol(gridding_parameters *p) { return grid_call(p, MODE_2, true, true); }
+int grid_4_cor(gridding_parameters *p) { return grid_call(p, MODE_4, false, false); }
+int facet_4_cor(gridding_parameters *p) { return grid_call(p, MODE_4, true, true); }
+int facet_4_cor_corrections(gridding_parameters *p) { return grid_call(p, MODE_4J, true, true); }
+int grid_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, false, false); }
+// wrapper.cpp:402-425: faceting uvw transform but no phase shift for the sampling function
+int facet_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, true, false); }
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+"""Convolution kernels and the grid-taper correction.
+
+Sampling layout follows bullseye's 1D AA filter array (bullseye/mo/convolution_filter.py:93-107
+and bullseye/mo/cpu_gpu_common/gridding_parameters.h:69-72): for half support S and oversampling
+OS the array has csz = (2S+3) + (2S+2)(OS-1) samples, centre index (S+1)*OS, sample spacing
+1/OS grid cells, and is normalised to unit sum.
+
+The reference only generates sinc x {unity, Kaiser-Bessel, Hamming}; the imaging configuration
+asks for a prolate spheroidal (PSWF) anti-aliasing kernel.  It is generated here from the
+zeroth-order prolate spheroidal angular function of the first kind (scipy.special.pro_ang1,
+Flammer's normalisation) with bandwidth parameter c = pi * W / 2 over the full width W = 2S+1,
+the classic gridding function of Schwab (1984) with alpha = 0.
+"""
+import numpy as np
+from scipy import special
+
+
+def aa_kernel_size(support, oversample):
+    """Number of samples of the oversampled 1D kernel (convolution_filter.py:93-95)."""
+    return (2 * support + 3) + (2 * support + 2) * (oversample - 1)
+
+
+def _positions(support, oversample):
+    csz = aa_kernel_size(support, oversample)
+    centre = (support + 1) * oversample
+    return (np.arange(csz) - centre) / float(oversample)
+
+
+def pswf_aa_kernel(support, oversample, dtype=np.float64):
+    """PSWF anti-aliasing kernel over 2S+1 cells, oversampled, unit sum."""
+    width = 2 * support + 1
+    x = _positions(support, oversample)
+    nu = x / (width / 2.0)
+    vals = np.zeros_like(x)
+    inside = np.abs(nu) < 1.0
+    vals[inside] = special.pro_ang1(0, 0, np.pi * width / 2.0, nu[inside])[0]
+    vals /= vals.sum()
+    return vals.astype(dtype)
+
+
+def sinc_aa_kernel(support, oversample, dtype=np.float64):
+    """bullseye's default 'sinc' AA filter (convolution_filter.py:96-100 with the unity window)."""
+    csz = aa_kernel_size(support, oversample)
+    centre = csz // 2
+    x = np.arange(-centre, centre + 1)
+    vals = np.sinc(x / float(oversample))
+    vals /= vals.sum()
+    return vals.astype(dtype)
+
+
+def kernel_tap_sums(conv, support, oversample):
+    """Sum of the 2S+1 taps used for each truncated fraction index f in [0, OS]."""
+    idx = np.arange(oversample + 1)[:, None] + oversample * np.arange(2 * support + 1)[None, :]
+    return np.asarray(conv, dtype=np.float64)[idx].sum(axis=1)
+
+
+def grid_taper(conv, support, oversample, n):
+    """Image-plane response of the gridding kernel along one axis.
+
+    Returns T[k] for image pixel k = 0..n-1 in the order finalize produces (after fftshift,
+    pixel k sits at l = k - n//2 cycles per grid): the DFT of the oversampled kernel samples
+    taken at the pixel frequencies, T(l) = sum_i conv[i] cos(2 pi (i - c) l / (n OS)),
+    normalised so that T at l = 0 is 1.  Dividing an image by T(l) T(m) removes the kernel's
+    taper (the grid-taper correction).
+    """
+    conv = np.asarray(conv, dtype=np.float64)
+    centre = (support + 1) * oversample
+    off = (np.arange(conv.size) - centre) / float(oversample)
+    l = np.arange(n) - n // 2
+    t = np.cos(2.0 * np.pi * np.outer(l, off) / n) @ conv
+    return t / conv.sum()
+
+
+def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_support=None,
+                  dtype=np.complex128):
+    """2D w-projection kernels in the reference's layout complex[wplanes][csz][csz].
+
+    Plane p holds the uv-plane kernel for w_p = p * wmax / (wplanes - 1) (the reference's plane
+    choice is lrint(|w| / wmax * (wplanes - 1)), convolution_policies.h:264).  Each kernel is the
+    Fourier transform of the image-plane product PSWF_taper(l) PSWF_taper(m) e^{-2 pi i w_p (n-1)}
+    evaluated on an oversampled image grid, cut to the (2S+3)-cell padded support, and
+    normalised so the zero-w plane sums to one (as the AA kernel).
+    """
+    aa_support = support if aa_support is None else aa_support
+    csz = aa_kernel_size(support, oversample)
+    # image-plane sampling: the kernel spans csz samples at 1/OS cell spacing; its transform
+    # is sampled on an image grid of csz points covering OS times the field of view
+    fov = nx * cell_arcsec * np.pi / 648000.0
+    k = np.arange(csz) - csz // 2
+    lm = k / float(csz) * fov * oversample
+    aa = pswf_aa_kernel(aa_support, oversample)
+    aa_pad = np.zeros(csz)
+    a0 = csz // 2 - aa.size // 2
+    aa_pad[a0:a0 + aa.size] = aa
+    taper1 = np.real(np.fft.fftshift(np.fft.fft(np.fft.ifftshift(aa_pad))))
+    taper2 = np.outer(taper1, taper1)
+    lg, mg = np.meshgrid(lm, lm, indexing="ij")
+    r2 = np.minimum(lg ** 2 + mg ** 2, 0.99)
+    nm1 = np.sqrt(1.0 - r2) - 1.0
+    out = np.empty((wplanes, csz, csz), dtype=np.complex128)
+    step = wmax / float(max(wplanes - 1, 1))
+    for p in range(wplanes):
+        img = taper2 * np.exp(-2j * np.pi * (p * step) * nm1)
+        ker = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(img)))
+        out[p] = ker
+    out /= out[0].real.sum()
+    return out.astype(dtype)

@@ -1,0 +1,225 @@
+/*
+ * bullseye_amd.h -- C ABI of the MI355X (gfx950) imaging library.
+ *
+ * The library is a drop-in for bullseye's imaging libraries
+ * (bullseye/mo/cpu_gpu_common/wrapper.h:41-61): the same 19 extern "C" symbols taking the
+ * same `gridding_parameters` struct (bullseye/mo/cpu_gpu_common/gridding_parameters.h:44-109),
+ * field for field, so bullseye's ctypes mirror (bullseye/mo/gridding_parameters.py:46-112)
+ * drives it unchanged.  Three builds come from one source tree:
+ *
+ *   libmi355x_imaging32.so     -DBULLSEYE_SINGLE  struct/pointee types of libgpu_imaging32.so
+ *   libmi355x_imaging64.so     -DBULLSEYE_DOUBLE  struct/pointee types of libgpu_imaging64.so
+ *   libmi355x_imaging_mixed.so -DBULLSEYE_MIXED   uvw/geometry float64, data float32
+ *                                                  (struct layout identical to DOUBLE)
+ *
+ * Precision typedefs follow bullseye/mo/cpu_gpu_common/base_types.h:41-59; MIXED is new.
+ *
+ * Every entry point returns an int status (0 = success).  The reference declares them void;
+ * its callers ignore the return value (bullseye/bin/bullseye_pipeliner.py:306-424 leaves
+ * ctypes' default int restype), so an int return is invisible to them.  On failure the
+ * message is available from bullseye_last_error(); nothing throws across the ABI.
+ */
+#ifndef BULLSEYE_AMD_H
+#define BULLSEYE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#if defined(BULLSEYE_SINGLE)
+typedef float visibility_base_type;
+typedef float uvw_base_type;
+typedef float reference_wavelengths_base_type;
+typedef float convolution_base_type;
+typedef float visibility_weights_base_type;
+typedef float grid_base_type;
+typedef double normalization_base_type;
+#elif defined(BULLSEYE_DOUBLE)
+typedef double visibility_base_type;
+typedef double uvw_base_type;
+typedef double reference_wavelengths_base_type;
+typedef double convolution_base_type;
+typedef double visibility_weights_base_type;
+typedef double grid_base_type;
+typedef double normalization_base_type;
+#elif defined(BULLSEYE_MIXED)
+typedef float visibility_base_type;
+typedef double uvw_base_type;
+typedef double reference_wavelengths_base_type;
+typedef float convolution_base_type;
+typedef float visibility_weights_base_type;
+typedef float grid_base_type;
+typedef double normalization_base_type;
+#else
+#error "define one of BULLSEYE_SINGLE, BULLSEYE_DOUBLE, BULLSEYE_MIXED"
+#endif
+
+/*
+ * Field order and widths are those of bullseye/mo/cpu_gpu_common/gridding_parameters.h:44-109.
+ * Pointee layouts (C order):
+ *   visibilities           complex<visibility_base_type>[row][channel][correlation]
+ *   uvw_coords             uvw_base_type[row][3]          metres (u, v, w)
+ *   reference_wavelengths  reference_wavelengths_base_type[spw][channel]   metres
+ *   visibility_weights     visibility_weights_base_type[row][channel][correlation]
+ *   flags                  bool[row][channel][correlation]
+ *   flagged_rows, field_array, spw_index_array: per row
+ *   conv  AA (wplanes <= 1): convolution_base_type[(2S+3)+(2S+2)(OS-1)]
+ *         w-projection:      complex<convolution_base_type>[wplanes][csz][csz]
+ *   output_buffer          complex<grid_base_type>[facet][cube channel][pol gridded][ny][nx]
+ *   facet_centres          uvw_base_type[facet][2]        RA, Dec in arcsec
+ *   jones_terms            complex<visibility_base_type>[time][antenna][facet][spw][channel][4]
+ *   sampling_function_buffer complex<grid_base_type>[facet][sf channel][ny][nx]
+ *   baseline_starting_indexes size_t[baseline_count + 1]
+ *   normalization_terms    written by initLibrary: normalization_base_type[facet][cube][pol]
+ */
+typedef struct gridding_parameters {
+    void *visibilities;
+    uvw_base_type *uvw_coords;
+    reference_wavelengths_base_type *reference_wavelengths;
+    visibility_weights_base_type *visibility_weights;
+    bool *flags;
+    bool *flagged_rows;
+    unsigned int *field_array;
+    unsigned int *spw_index_array;
+    unsigned int imaging_field;
+    size_t baseline_count;
+    size_t row_count;
+    size_t chunk_max_row_count;
+    size_t channel_count;
+    size_t number_of_polarization_terms;
+    size_t number_of_polarization_terms_being_gridded;
+    size_t spw_count;
+    size_t no_timestamps_read;
+    size_t nx;
+    size_t ny;
+    uvw_base_type cell_size_x;
+    uvw_base_type cell_size_y;
+    convolution_base_type *conv;
+    size_t conv_support;
+    size_t conv_oversample;
+    size_t polarization_index;
+    size_t second_polarization_index;
+    void *output_buffer;
+    uvw_base_type phase_centre_ra;
+    uvw_base_type phase_centre_dec;
+    uvw_base_type *facet_centres;
+    size_t num_facet_centres;
+    void *jones_terms;
+    bool should_invert_jones_terms;
+    unsigned int *antenna_1_ids;
+    unsigned int *antenna_2_ids;
+    size_t *timestamp_ids;
+    size_t antenna_count;
+    bool *enabled_channels;
+    size_t *channel_grid_indicies;
+    size_t cube_channel_dim_size;
+    bool should_grid_sampling_function;
+    void *sampling_function_buffer;
+    size_t *sampling_function_channel_grid_indicies;
+    size_t sampling_function_channel_count;
+    bool is_final_data_chunk;
+    size_t wplanes;
+    uvw_base_type wmax_est;
+    size_t *baseline_starting_indexes;
+    size_t *antenna_jones_starting_indexes;
+    size_t *jones_time_indicies_per_antenna;
+    normalization_base_type *normalization_terms;
+} gridding_parameters;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- the 19 entry points of bullseye/mo/cpu_gpu_common/wrapper.h:41-61 ---------------- */
+
+/* wrapper.h:42 / cpu_algorithm/wrapper.cpp:66-69: seconds spent in grid_* and facet_* calls
+ * (device time of the gridding stream, hipEvent-timed). */
+double get_gridding_walltime(void);
+/* wrapper.h:43 / cpu_algorithm/wrapper.cpp:70-73: seconds spent in finalize/finalize_psf. */
+double get_inversion_walltime(void);
+/* wrapper.h:44 / cpu_algorithm/wrapper.cpp:74-78: block until queued gridding has finished;
+ * copies results to host buffers if the final chunk was gridded. */
+void gridding_barrier(void);
+/* wrapper.h:45 / cpu_algorithm/wrapper.cpp:79-106, gpu_algorithm/wrapper.cu:81-232:
+ * one-time set-up; allocates normalization_terms and writes the pointer into params. */
+int initLibrary(gridding_parameters *params);
+/* wrapper.h:46 / cpu_algorithm/wrapper.cpp:107-115 */
+int releaseLibrary(void);
+/* wrapper.h:47 / cpu_algorithm/wrapper.cpp:116-135: uv-plane uniform weighting from the
+ * gridded sampling function. */
+int weight_uniformly(gridding_parameters *params);
+/* wrapper.h:48 / cpu_algorithm/wrapper.cpp:136-163: grid /= normalization per slice. */
+int normalize(gridding_parameters *params);
+/* wrapper.h:49 / gpu_algorithm/wrapper.cu:367-454: in-place reorder of the row arrays by
+ * baseline (time order kept) and prefix scan into baseline_starting_indexes. */
+int repack_input_data(gridding_parameters *params);
+/* wrapper.h:50 / fft_and_repacking_routines.cpp:83-118: ifftshift, unnormalised backward
+ * c2c FFT, fftshift and float32 real part written in place. */
+int finalize(gridding_parameters *params);
+/* wrapper.h:51 / fft_and_repacking_routines.cpp:119-150: same for the sampling function. */
+int finalize_psf(gridding_parameters *params);
+/* wrapper.h:52-60 / cpu_algorithm/wrapper.cpp:182-425: grid one chunk of rows. grid_* grid
+ * at the phase centre; facet_* rotate phase and uvw to each facet centre.  These return
+ * once the work is queued; call gridding_barrier() before touching the buffers. */
+int grid_single_pol(gridding_parameters *params);
+int facet_single_pol(gridding_parameters *params);
+int grid_duel_pol(gridding_parameters *params);
+int facet_duel_pol(gridding_parameters *params);
+int grid_4_cor(gridding_parameters *params);
+int facet_4_cor(gridding_parameters *params);
+int facet_4_cor_corrections(gridding_parameters *params);
+int grid_sampling_function(gridding_parameters *params);
+int facet_sampling_function(gridding_parameters *params);
+
+/* ---- extensions (no counterpart in the reference) -------------------------------------- */
+
+/* Message of the last failing call ("" if none). */
+const char *bullseye_last_error(void);
+
+/* Compute path of the gridding kernels. */
+enum bullseye_compute_path {
+    BULLSEYE_PATH_DEFAULT = 0, /* FP64 for the DOUBLE build, FP32_KAHAN otherwise */
+    BULLSEYE_PATH_FP64 = 1,    /* FP64 geometry, taps, register accumulators and grid */
+    BULLSEYE_PATH_FP32_KAHAN = 2, /* FP64 phase argument + uvw geometry, FP32 taps with
+                                     compensated register sums, FP64 grid flushes */
+    BULLSEYE_PATH_FP32 = 3     /* as FP32_KAHAN without the compensation term */
+};
+
+/* Library-wide options.  The reference struct has no spare fields, so options live beside
+ * it; callers that never set them get the reference's behaviour. */
+typedef struct bullseye_options {
+    uint32_t struct_size;       /* sizeof(bullseye_options) */
+    uint32_t device_resident;   /* 1: every array pointer in gridding_parameters (inputs,
+                                   output_buffer, sampling_function_buffer) is device memory;
+                                   no host<->device copies are made */
+    uint32_t compute_path;      /* enum bullseye_compute_path */
+    uint32_t apply_taper_correction; /* 1: finalize divides each facet image by the grid
+                                   taper (DFT of the sampled AA kernel) */
+    uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto) */
+    uint32_t rows_per_stream;   /* Romein stream: time rows walked per work item (0 = auto) */
+    uint32_t reserved[10];
+} bullseye_options;
+
+int bullseye_set_options(const bullseye_options *options);
+int bullseye_get_options(bullseye_options *options);
+
+/* Zero the device accumulation grids and normalisation sums so a new imaging pass can
+ * start without releaseLibrary/initLibrary. */
+int bullseye_reset_grids(void);
+
+/* Counters of the last gridding call (filled after gridding_barrier). */
+typedef struct bullseye_stats {
+    uint64_t samples_gridded;   /* (facet, row, channel) samples that were in-grid and unflagged */
+    uint64_t taps;              /* sum over those samples of the tap count (2S+1)^2 */
+    double kernel_ms;           /* device time of the gridding kernel(s) of the call */
+    double prep_ms;             /* device time of ingest (baseline sort) of the call */
+    double copy_ms;             /* device time of host->device copies of the call */
+} bullseye_stats;
+int bullseye_get_stats(bullseye_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

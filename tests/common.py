@@ -1,0 +1,104 @@
+"""Shared helpers for the parity tests: small seeded observations, identical inputs for the
+library and the oracle, and the error measures (relative to the oracle's peak)."""
+import ctypes
+
+import numpy as np
+
+from bullseye_amd import kernels, synth
+from bullseye_amd.base_types import force_precision
+from bullseye_amd.binding import Binding, ImagingSpec, jones_reference_layout
+from bullseye_amd.library_loader import bullseye_options, load_library
+
+MODES_1 = ("grid_single_pol", "facet_single_pol")
+MODES_2 = ("grid_duel_pol", "facet_duel_pol")
+MODES_4 = ("grid_4_cor", "facet_4_cor", "facet_4_cor_corrections")
+MODES_PSF = ("grid_sampling_function", "facet_sampling_function")
+ALL_MODES = MODES_1 + MODES_2 + MODES_4 + MODES_PSF
+
+
+def mode_ncorr(mode):
+    return 4 if mode in MODES_4 else 2 if mode in MODES_2 else 1
+
+
+def small_obs(mode="grid_single_pol", seed=0, n_ant=12, n_time=30, n_chan=8, size_m=1000.0,
+              flag_fraction=0.05):
+    n_corr = 4 if mode in MODES_4 or mode in MODES_2 else 1
+    jones = mode == "facet_4_cor_corrections"
+    return synth.simulate(n_ant=n_ant, n_time=n_time, n_chan=n_chan, array_size_m=size_m,
+                          n_corr=n_corr, jones_sigma=0.05 if jones else None, seed=seed,
+                          flag_fraction=flag_fraction)
+
+
+def round_to(obs, precision):
+    """Round the observation's arrays to the precision build's dtypes, so the library and the
+    FP64 oracle see identical input values."""
+    bt = force_precision(precision)
+    obs.uvw = obs.uvw.astype(bt.uvw_type).astype(np.float64)
+    obs.vis = obs.vis.astype(bt.visibility_type).astype(np.complex64) \
+        if bt.visibility_type == np.complex64 else obs.vis
+    obs.weights = obs.weights.astype(bt.weight_type).astype(np.float32)
+    obs.wl = obs.wavelengths.astype(bt.reference_wavelength_type).astype(np.float64)
+    return obs
+
+
+def make_spec(obs, mode, npix=128, cell=20.0, support=3, oversample=63, n_facets=3,
+              precision="double", cube=1, seed=1):
+    bt = force_precision(precision)
+    conv = kernels.pswf_aa_kernel(support, oversample).astype(bt.fir_type).astype(np.float64)
+    ncorr = mode_ncorr(mode)
+    facets = None
+    F = 1
+    if mode.startswith("facet"):
+        rng = np.random.default_rng(seed)
+        facets = synth.facet_centres(n_facets, 1.0, obs.phase_centre, rng)
+        facets = facets.astype(bt.uvw_type).astype(np.float64)
+        F = n_facets
+    spec = ImagingSpec(npix=npix, cell=cell, conv=conv, support=support, oversample=oversample,
+                       facet_centres=facets, n_pol_gridded=ncorr, cube=cube)
+    if ncorr == 2:
+        spec.pol_index, spec.second_pol_index = 0, 3
+    if cube > 1:
+        spec.channel_grid_index = (np.arange(obs.n_chan) % cube).astype(np.uint64)
+    if mode == "facet_4_cor_corrections":
+        spec.jones = jones_reference_layout(obs, F, np.complex128)
+    if mode in MODES_PSF:
+        spec.sf_count = cube
+        spec.sf_channel_grid_index = spec.channel_grid_index
+    return spec
+
+
+def set_options(lib, **kw):
+    o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    rc = lib.bullseye_set_options(ctypes.byref(o))
+    assert rc == 0, lib.bullseye_last_error()
+
+
+def run_library(lib, binding, mode, normalize=True, chunks=None):
+    """init -> grid (one or several chunks) -> barrier -> normalize -> release."""
+    rc = lib.initLibrary(binding.byref())
+    assert rc == 0, lib.bullseye_last_error().decode()
+    try:
+        fn = getattr(lib, mode)
+        rc = fn(binding.byref())
+        assert rc == 0, lib.bullseye_last_error().decode()
+        lib.gridding_barrier()
+        norm = np.ctypeslib.as_array(
+            ctypes.cast(binding.params.normalization_terms, ctypes.POINTER(ctypes.c_double)),
+            (binding.params.num_facet_centres * binding.params.cube_channel_dim_size *
+             binding.params.number_of_polarization_terms_being_gridded,)).copy()
+        if normalize and mode not in MODES_PSF:
+            rc = lib.normalize(binding.byref())
+            assert rc == 0, lib.bullseye_last_error().decode()
+    finally:
+        lib.releaseLibrary()
+    return norm
+
+
+def rel_err(got, want):
+    got = np.asarray(got, dtype=np.complex128)
+    want = np.asarray(want, dtype=np.complex128)
+    peak = np.abs(want).max()
+    d = np.abs(got - want)
+    return float(np.sqrt(np.mean(d ** 2)) / peak), float(d.max() / peak)

@@ -1,0 +1,122 @@
+"""HIP library vs the C oracle on identical seeded inputs, through the C ABI.
+
+Bars (relative to the oracle's peak): FP64 path into a float64 grid (DOUBLE build) rms <= 1e-11,
+peak <= 1e-9; FP64 path into a float32 grid (MIXED/SINGLE builds) rms <= 2e-7, peak <= 1e-6 (the
+output rounding); FP32 paths rms <= 1e-6 (Kahan) / 2e-6 (plain), peak <= 2e-5.  Normalisation
+terms to 1e-12 relative (FP64 sums on every path).
+"""
+import numpy as np
+import pytest
+
+from common import (ALL_MODES, MODES_PSF, make_spec, rel_err, round_to, run_library,
+                    set_options, small_obs)
+from bullseye_amd.binding import Binding
+from bullseye_amd.library_loader import (PATH_FP32, PATH_FP32_KAHAN, PATH_FP64, load_library)
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {PATH_FP64: (1e-11, 1e-9), PATH_FP32_KAHAN: (1e-6, 2e-5), PATH_FP32: (2e-6, 2e-5),
+       "fp64_float_grid": (2e-7, 1e-6)}
+
+
+def tolerance(precision, path):
+    if path == PATH_FP64 and precision != "double":
+        return TOL["fp64_float_grid"]
+    return TOL[path]
+CASES = [("double", PATH_FP64), ("mixed", PATH_FP32_KAHAN), ("mixed", PATH_FP64),
+         ("mixed", PATH_FP32), ("single", PATH_FP32_KAHAN)]
+
+
+def oracle_grid(obs, spec, mode):
+    ref = Binding("double", obs, spec, with_psf=mode in MODES_PSF)
+    norm = oracle.run(ref, mode)
+    out = ref.psf if mode in MODES_PSF else ref.output
+    return out, norm
+
+
+@pytest.mark.parametrize("precision,path", CASES)
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_mode_parity(precision, path, mode):
+    print("case", precision, path, mode, flush=True)
+    obs = round_to(small_obs(mode, seed=3), precision)
+    spec = make_spec(obs, mode, precision=precision)
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", precision)
+    set_options(lib, compute_path=path)
+    b = Binding(precision, obs, spec, with_psf=mode in MODES_PSF)
+    norm = run_library(lib, b, mode)
+    got = b.psf if mode in MODES_PSF else b.output
+    rms, peak = rel_err(got, want)
+    print("rms %.3e peak %.3e" % (rms, peak))
+    assert np.abs(want).max() > 0
+    tol = tolerance(precision, path)
+    assert rms <= tol[0] and peak <= tol[1], (rms, peak)
+    if mode not in MODES_PSF:
+        np.testing.assert_allclose(norm, want_norm, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["facet_single_pol", "facet_4_cor_corrections"])
+def test_device_resident(mode):
+    import torch
+    print("case device_resident", mode, flush=True)
+    obs = round_to(small_obs(mode, seed=5), "mixed")
+    spec = make_spec(obs, mode, precision="mixed")
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", "mixed")
+    set_options(lib, compute_path=PATH_FP32_KAHAN, device_resident=1)
+    b = Binding("mixed", obs, spec, device=torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    try:
+        norm = run_library(lib, b, mode)
+    finally:
+        set_options(lib)
+    rms, peak = rel_err(b.host("output_buffer"), want)
+    assert rms <= 1e-6 and peak <= 2e-5, (rms, peak)
+    np.testing.assert_allclose(norm, want_norm, rtol=1e-12)
+
+
+def test_cube_channels_and_chunks():
+    """Per-channel grid slices (cube > 1) and two row chunks accumulated into one grid."""
+    print("case cube+chunks", flush=True)
+    mode = "facet_single_pol"
+    obs = round_to(small_obs(mode, seed=9), "mixed")
+    spec = make_spec(obs, mode, precision="mixed", cube=4)
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", "mixed")
+    set_options(lib, compute_path=PATH_FP64)
+    half = obs.rows // 2
+    b1 = Binding("mixed", obs, spec, rows=(0, half))
+    b2 = Binding("mixed", obs, spec, rows=(half, obs.rows))
+    b1.params.is_final_data_chunk = False
+    assert lib.initLibrary(b1.byref()) == 0
+    try:
+        assert lib.facet_single_pol(b1.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        b2.params.normalization_terms = b1.params.normalization_terms
+        b2.params.output_buffer = b1.params.output_buffer
+        assert lib.facet_single_pol(b2.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        assert lib.normalize(b2.byref()) == 0
+    finally:
+        lib.releaseLibrary()
+    rms, peak = rel_err(b1.output, want)
+    assert rms <= 2e-7 and peak <= 1e-6, (rms, peak)
+
+
+def test_singular_jones_reports_error():
+    print("case singular jones", flush=True)
+    mode = "facet_4_cor_corrections"
+    obs = small_obs(mode, seed=2, n_ant=6, n_time=4, n_chan=2)
+    spec = make_spec(obs, mode, precision="mixed")
+    spec.jones[1, 2] = 0
+    lib = load_library("GPU", "mixed")
+    set_options(lib)
+    b = Binding("mixed", obs, spec)
+    assert lib.initLibrary(b.byref()) == 0
+    try:
+        rc = lib.facet_4_cor_corrections(b.byref())
+        assert rc != 0
+        assert b"SINGULAR" in lib.bullseye_last_error()
+    finally:
+        lib.releaseLibrary()
