@@ -1,0 +1,203 @@
+"""Benchmark: Gtaps/s of gridding + phase rotation, FP32-compensated vs FP64, on one MI355X.
+
+Workload (one GPU): targeted faceting with 4 correlations -- 64 antennas uniform in a 3 km disc
+(2016 baselines), 240 timesteps over 8 h at dec -30 deg, 64 channels 1.40-1.42 GHz, 4
+correlations complex64 corrupted by per-antenna per-timestep Jones I + N(0, 0.05^2); 25 facets
+of 512^2 at 2" within 2 deg of the phase centre; 7x7 PSWF AA kernel oversampled x63; gridded
+with facet_4_cor_corrections (phase rotation, uvw rotation, Jones correction).  Inputs are
+synthetic (seeded) and resident in HBM before timing.
+
+A step is one imaging pass: zero the accumulation grids, grid every row (facet_4_cor_corrections,
+device-resident buffers) and publish the grids to the caller's buffer.  One tap is one
+(facet, row, channel, kernel point) accumulate of an unflagged in-grid sample (4 correlations
+each).  The FP64 path runs the same step on the same inputs; the error of the FP32-compensated
+grids is stated against it (full size) and against the FP64 CPU oracle (a row sample).
+
+Prints one JSON line.  --gpus N with N > 1 exits: every box here has one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gtaps/s gridding+phase-rot, FP32-comp vs FP64 on 1 GPU at stated RMS error"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_TAP_PER_CORR = 5   # conv_u*conv_v, complex x real, complex add (real AA kernel)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample-rows", type=int, default=0,
+                    help="rows of the oracle's CPU sample (0 = automatic)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--channels-per-stream", type=int, default=0)
+    ap.add_argument("--rows-per-stream", type=int, default=0)
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_errors(got, want):
+    import numpy as np
+    out = []
+    for f in range(want.shape[0]):
+        w = want[f].astype(np.complex128)
+        d = np.abs(got[f].astype(np.complex128) - w)
+        peak = np.abs(w).max()
+        out.append((float(np.sqrt(np.mean(d ** 2)) / peak), float(d.max() / peak)))
+    return max(e[0] for e in out), max(e[1] for e in out)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: only --gpus 1 is supported (each box has one GPU; the facet-sharded "
+              "multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from bullseye_amd import kernels, synth
+    from bullseye_amd.binding import Binding, ImagingSpec, jones_reference_layout
+    from bullseye_amd.library_loader import (PATH_FP32_KAHAN, PATH_FP64, bullseye_options,
+                                             bullseye_stats, load_library)
+
+    cfg = synth.CONFIGS["c1"]
+    im = cfg["image"]
+    t0 = time.time()
+    obs = synth.simulate(**cfg["sim"], seed=args.seed)
+    rng = np.random.default_rng(args.seed + 100)
+    facets = synth.facet_centres(im["facets"], im["facet_radius_deg"], obs.phase_centre, rng)
+    conv = kernels.pswf_aa_kernel(im["support"], im["oversample"])
+    spec = ImagingSpec(npix=im["npix"], cell=im["cell"], conv=conv, support=im["support"],
+                       oversample=im["oversample"], facet_centres=facets, n_pol_gridded=4,
+                       jones=jones_reference_layout(obs, im["facets"], np.complex64))
+    log("synthesised %d rows x %d ch x %d corr in %.1fs" % (obs.rows, obs.n_chan, obs.n_corr,
+                                                           time.time() - t0))
+    dev = torch.device("cuda:0")
+    lib = load_library("GPU", "mixed")
+    b = Binding("mixed", obs, spec, device=dev)
+    torch.cuda.synchronize()
+
+    def set_opts(path):
+        o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=1,
+                             compute_path=path, channels_per_stream=args.channels_per_stream,
+                             rows_per_stream=args.rows_per_stream)
+        assert lib.bullseye_set_options(ctypes.byref(o)) == 0, lib.bullseye_last_error()
+
+    def check(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (what, lib.bullseye_last_error().decode()))
+
+    set_opts(PATH_FP32_KAHAN)
+    check(lib.initLibrary(b.byref()), "initLibrary")
+    stats = bullseye_stats()
+
+    def run(path, steps, warmup):
+        set_opts(path)
+        for _ in range(warmup):
+            check(lib.bullseye_reset_grids(), "reset")
+            check(lib.facet_4_cor_corrections(b.byref()), "grid")
+            lib.gridding_barrier()
+        torch.cuda.synchronize()
+        kms, taps = [], 0
+        t = time.perf_counter()
+        for _ in range(steps):
+            check(lib.bullseye_reset_grids(), "reset")
+            check(lib.facet_4_cor_corrections(b.byref()), "grid")
+            lib.gridding_barrier()
+            lib.bullseye_get_stats(ctypes.byref(stats))
+            kms.append(stats.kernel_ms)
+            taps = stats.taps
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / steps
+        return wall, taps, float(np.mean(kms)), stats.samples_gridded
+
+    wall32, taps, k32, nsamp = run(PATH_FP32_KAHAN, args.steps, args.warmup)
+    grid32 = b.host("output_buffer").copy()
+    log("FP32-comp: %.2f ms/step, kernel %.2f ms, %.3e taps" % (wall32 * 1e3, k32, taps))
+    wall64, taps64, k64, _ = run(PATH_FP64, max(1, args.steps // 2), 1)
+    grid64 = b.host("output_buffer").copy()
+    log("FP64: %.2f ms/step, kernel %.2f ms" % (wall64 * 1e3, k64))
+    lib.releaseLibrary()
+    rms_vs_gpu64, peak_vs_gpu64 = rel_errors(grid32.reshape(im["facets"], -1),
+                                             grid64.reshape(im["facets"], -1))
+
+    gtaps32 = taps / wall32 / 1e9
+    gtaps64 = taps64 / wall64 / 1e9
+    flops = taps * 4 * FLOPS_PER_TAP_PER_CORR
+    achieved_tflops = flops / (k32 * 1e-3) / 1e12
+
+    result = {
+        "metric": METRIC, "value": round(gtaps32, 3), "unit": "Gtaps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall32 * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded; inputs resident in HBM)",
+        "config": {"workload": "targeted faceting, 4 correlations with Jones: 64 ant (2016 "
+                               "baselines) x 240 t x 64 ch, 25 facets of 512^2 at 2\", 7x7 "
+                               "PSWF x63 (facet_4_cor_corrections)",
+                   "rows": obs.rows, "channels": obs.n_chan, "correlations": 4,
+                   "facets": im["facets"], "npix": im["npix"], "taps_per_step": taps,
+                   "samples_per_step": nsamp},
+        "fp64_gtaps_per_s": round(gtaps64, 3), "fp32comp_over_fp64": round(gtaps32 / gtaps64, 3),
+        "rms_err_vs_fp64_gpu": rms_vs_gpu64, "peak_err_vs_fp64_gpu": peak_vs_gpu64,
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel_ms": round(k32, 3),
+                     "flops_per_launch": flops},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle
+        n = args.cpu_sample_rows or (obs.rows // 20)
+        sub = (0, n)
+        spec64 = ImagingSpec(**{**spec.__dict__, "jones": jones_reference_layout(
+            obs, im["facets"], np.complex128)})
+        ref = Binding("double", obs, spec64, rows=sub)
+        t = time.perf_counter()
+        oracle.run(ref, "facet_4_cor_corrections", normalize=False)
+        t_or64 = time.perf_counter() - t
+        # GPU FP32-comp on the same rows
+        set_opts(PATH_FP32_KAHAN)
+        bs = Binding("mixed", obs, spec, rows=sub, device=dev)
+        torch.cuda.synchronize()
+        check(lib.initLibrary(bs.byref()), "initLibrary(sample)")
+        check(lib.facet_4_cor_corrections(bs.byref()), "grid(sample)")
+        lib.gridding_barrier()
+        lib.bullseye_get_stats(ctypes.byref(stats))
+        sample_taps = stats.taps
+        lib.releaseLibrary()
+        rms_o, peak_o = rel_errors(bs.host("output_buffer").reshape(im["facets"], -1),
+                                   ref.output.reshape(im["facets"], -1))
+        # CPU baseline: the single-precision restatement on the same sample, all host cores
+        ref32 = Binding("single", obs, spec, rows=sub)
+        t = time.perf_counter()
+        oracle.run(ref32, "facet_4_cor_corrections", precision="single", normalize=False)
+        t_or32 = time.perf_counter() - t
+        threads = oracle.load("single").oracle_max_threads()
+        result["rms_err_vs_fp64_oracle"] = rms_o
+        result["peak_err_vs_fp64_oracle"] = peak_o
+        result["cpu_baseline"] = {
+            "value": round(sample_taps / t_or32 / 1e9, 4), "unit": "Gtaps/s",
+            "cores": min(threads, im["facets"]), "kind": "port",
+            "sample": "first %d of %d rows (%d timesteps) of the same workload, %.2e taps, "
+                      "%.1f s; OpenMP over facets (reference scheme)" % (
+                          n, obs.rows, n // 2016, sample_taps, t_or32)}
+        result["fp64_oracle_cpu_gtaps_per_s"] = round(sample_taps / t_or64 / 1e9, 4)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

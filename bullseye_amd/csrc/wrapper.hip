@@ -77,7 +77,7 @@ struct State {
     bool psf = false;
     int path = PATH_FP64;
     // device buffers
-    DevBuf grid, psfgrid, norm, counters, errflag, conv, tapsum, geom, taper;
+    DevBuf grid, psfgrid, norm, counters, errflag, conv_f, conv_d, tapsum, geom, taper;
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
@@ -189,8 +189,9 @@ static int barrier_impl() {
         (void)hipEventDestroy(c.k1);
         (void)hipEventDestroy(c.stop);
     }
+    const bool had_pending = !g.pending.empty();
     g.pending.clear();
-    if (g.counters.p) {
+    if (had_pending && g.counters.p) {
         unsigned long long c[2];
         e = hipMemcpy(c, g.counters.p, sizeof c, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail("reading counters: %s", hipGetErrorString(e));
@@ -402,7 +403,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.perm = g.perm.as<uint32_t>();
     a.bl_start = g.blstart.as<uint32_t>();
     a.geom = g.geom.as<FacetGeom>();
-    a.conv = g.conv.p;
+    a.conv = g.path == PATH_FP64 ? g.conv_d.p : g.conv_f.p;
     a.tapsum = g.tapsum.as<double>();
     a.grid = psf ? g.psfgrid.as<double>() : g.grid.as<double>();
     a.norm = g.norm.as<double>();
@@ -473,8 +474,13 @@ int bullseye_set_options(const bullseye_options *o) {
         return fail("bullseye_options.struct_size %u != %zu", o->struct_size, sizeof(bullseye_options));
     if (o->compute_path > BULLSEYE_PATH_FP32) return fail("bad compute_path %u", o->compute_path);
     g.opts = *o;
-    if (g.initialized && o->compute_path)
-        g.path = (int)o->compute_path;
+    if (g.initialized)
+        g.path = o->compute_path ? (int)o->compute_path
+#if defined(BULLSEYE_DOUBLE)
+                                 : PATH_FP64;
+#else
+                                 : PATH_F32K;
+#endif
     return 0;
 }
 
@@ -586,14 +592,12 @@ int initLibrary(gridding_parameters *p) {
         std::vector<double> ts(g.OS + 1, 0.0);
         for (size_t f = 0; f <= g.OS; ++f)
             for (size_t k = 0; k < 2 * g.S + 1; ++k) ts[f] += cd[f + k * g.OS];
-        const bool fp64 = g.path == PATH_FP64;
-        HIP_TRY(g.conv.ensure(g.csz * 8));
-        if (fp64) {
-            HIP_TRY(hipMemcpy(g.conv.p, cd.data(), g.csz * 8, hipMemcpyHostToDevice));
-        } else {
-            std::vector<float> cf(cd.begin(), cd.end());
-            HIP_TRY(hipMemcpy(g.conv.p, cf.data(), g.csz * 4, hipMemcpyHostToDevice));
-        }
+        // both precisions, so the compute path can change after initLibrary
+        std::vector<float> cf(cd.begin(), cd.end());
+        HIP_TRY(g.conv_d.ensure(g.csz * 8));
+        HIP_TRY(g.conv_f.ensure(g.csz * 4));
+        HIP_TRY(hipMemcpy(g.conv_d.p, cd.data(), g.csz * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.conv_f.p, cf.data(), g.csz * 4, hipMemcpyHostToDevice));
         HIP_TRY(g.tapsum.ensure((g.OS + 1) * 8));
         HIP_TRY(hipMemcpy(g.tapsum.p, ts.data(), (g.OS + 1) * 8, hipMemcpyHostToDevice));
         g.taper_x = taper_vector(cd, g.S, g.OS, g.nx);
@@ -613,7 +617,8 @@ int initLibrary(gridding_parameters *p) {
 int releaseLibrary(void) {
     if (!g.initialized) return 0;
     (void)barrier_impl();
-    DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv, &g.tapsum,
+    DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv_f,
+                      &g.conv_d, &g.tapsum,
                       &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
                       &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
