@@ -1,0 +1,412 @@
+// romein_w.hip -- Romein gridding specialised on the kernel's full support W (3..11).
+//
+// Same decomposition and semantics as romein.hip, with three changes for the common case:
+//  * W is a compile-time constant, so residues and ownership are constant-folded;
+//  * the lane that prepares a sample also writes the sample's taps *rotated to residue
+//    order* into its LDS record (cu[a] is the u-tap for the lane that owns x = a mod W, cv[j]
+//    the v-tap for the accumulator that owns y = j mod W), so the walk is loads + MACs;
+//  * 4-correlation modes split the correlations over two lanes (CPL = 2 per lane), which
+//    halves the accumulator registers per lane.
+// A lane group is W * NSPLIT lanes: lane (h, a) owns u-residue a and correlations
+// [h*CPL, (h+1)*CPL) of all W v-residues.  When a sample leaves the footprint where it was
+// (x0, y0, grid slice) unchanged, no ownership check is needed (fast path).
+#include "device_common.h"
+
+namespace bam {
+
+namespace rw {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T> __device__ __forceinline__ cpx<T> cinv(cpx<T> d) {
+    T den = d.re * d.re + d.im * d.im;
+    return {d.re / den, -d.im / den};
+}
+
+template <typename T>
+__device__ __forceinline__ void jinv(const cpx<T> m[4], cpx<T> o[4], int *err) {
+    cpx<T> det = {m[0].re * m[3].re - m[0].im * m[3].im - (m[1].re * m[2].re - m[1].im * m[2].im),
+                  m[0].re * m[3].im + m[0].im * m[3].re - (m[1].re * m[2].im + m[1].im * m[2].re)};
+    if (det.re == 0 && det.im == 0) atomicOr(err, 1);
+    cpx<T> di = cinv(det);
+    o[0] = cmul(m[3], di);
+    o[1] = cmul(cpx<T>{-m[1].re, -m[1].im}, di);
+    o[2] = cmul(cpx<T>{-m[2].re, -m[2].im}, di);
+    o[3] = cmul(m[0], di);
+}
+
+template <typename T, int NC, int W> struct Rec {
+    int x0, y0, cgi, pad;
+    cpx<T> vis[NC];
+    T cu[W];
+    T cv[W];
+};
+
+template <int MODE, int PATH, int W> struct GridderW {
+    static constexpr int NC = ModeTraits<MODE>::NCORR;
+    static constexpr int CPL = NC == 4 ? 2 : NC;      // correlations per lane
+    static constexpr int NSPLIT = NC / CPL;
+    static constexpr int G = W * NSPLIT;               // lanes per group
+    static constexpr int GPW = 64 / G;                 // groups per wavefront
+    typedef typename PathTraits<PATH>::acc_t T;
+    static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
+    typedef Rec<T, NC, W> R_t;
+
+    static __device__ __forceinline__ int mod_w(int v) {  // v in (-W, 2W)
+        v = v < 0 ? v + W : v;
+        return v >= W ? v - W : v;
+    }
+
+    // Prepare one sample; returns false when it contributes nothing.
+    static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom &g,
+                                                const T *sconv, uint32_t facet, uint32_t row,
+                                                uint32_t c, R_t &r, double nw[NC]) {
+        const uint32_t s = a.spw[row];
+        if (s >= a.spw_count) {
+            atomicOr(a.error_flag, 4);
+            return false;
+        }
+        const uint32_t fc = s * a.chan + c;
+        if (!a.enabled[fc]) return false;
+        const uint64_t cgi = a.cgi[fc];
+        if (cgi >= a.cube) {
+            atomicOr(a.error_flag, 4);
+            return false;
+        }
+        if (a.row_flags[row] || a.field[row] != a.imaging_field) return false;
+        const double rw = 1.0 / (double)a.wavelengths[fc];
+        double u = (double)a.uvw[3 * (size_t)row] * rw;
+        double v = (double)a.uvw[3 * (size_t)row + 1] * rw;
+        const double w = (double)a.uvw[3 * (size_t)row + 2] * rw;
+        cpx<T> vis[NC];
+        T wt[NC];
+        if constexpr (MODE == MODE_PSF) {
+            vis[0] = {T(1), T(0)};
+            wt[0] = T(1);
+        } else {
+            const size_t base = ((size_t)row * a.chan + c) * a.npol;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const size_t idx = base + a.pol_idx[k];
+                vis[k] = {(T)a.vis[2 * idx], (T)a.vis[2 * idx + 1]};
+                wt[k] = a.flags[idx] ? T(0) : (T)a.weights[idx];
+            }
+        }
+        if constexpr (MODE == MODE_4J) {  // V = P^-1 V (Q^-1)^H (correlation_gridding_policies.h:523-542)
+            const size_t t = a.time_ids[row];
+            if (t >= a.ntime || a.ant1[row] >= a.antenna_count || a.ant2[row] >= a.antenna_count) {
+                atomicOr(a.error_flag, 4);
+                return false;
+            }
+            const size_t j1 = ((((size_t)t * a.antenna_count + a.ant1[row]) * a.facets + facet) *
+                                   a.spw_count + s) * a.chan + c;
+            const size_t j2 = ((((size_t)t * a.antenna_count + a.ant2[row]) * a.facets + facet) *
+                                   a.spw_count + s) * a.chan + c;
+            cpx<T> P[4], Q[4], Pi[4], Qi[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                P[k] = {(T)a.jones[8 * j1 + 2 * k], (T)a.jones[8 * j1 + 2 * k + 1]};
+                Q[k] = {(T)a.jones[8 * j2 + 2 * k], (T)a.jones[8 * j2 + 2 * k + 1]};
+            }
+            jinv(P, Pi, a.error_flag);
+            jinv(Q, Qi, a.error_flag);
+            const cpx<T> h0 = conjg(Qi[0]), h1 = conjg(Qi[2]), h2 = conjg(Qi[1]), h3 = conjg(Qi[3]);
+            const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
+            const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));
+            const cpx<T> t2 = cadd(cmul(vis[2], h0), cmul(vis[3], h2));
+            const cpx<T> t3 = cadd(cmul(vis[2], h1), cmul(vis[3], h3));
+            vis[0] = cadd(cmul(Pi[0], t0), cmul(Pi[1], t2));
+            vis[1] = cadd(cmul(Pi[0], t1), cmul(Pi[1], t3));
+            vis[2] = cadd(cmul(Pi[2], t0), cmul(Pi[3], t2));
+            vis[3] = cadd(cmul(Pi[2], t1), cmul(Pi[3], t3));
+        }
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            any |= (wt[k] != T(0));
+            vis[k].re *= wt[k];
+            vis[k].im *= wt[k];
+        }
+        if (!any) return false;
+        if (a.phase) {  // phase_transform_policies.h:138-146, argument in turns, FP64
+            const double turns = u * g.dl + v * g.dm + w * g.dn;
+            const double fr = 2.0 * (turns - rint(turns));
+            T sn, cs;
+            if constexpr (PATH == PATH_FP64) {
+                double s_, c_;
+                sincospi(fr, &s_, &c_);
+                sn = s_;
+                cs = c_;
+            } else {
+                float s_, c_;
+                sincospif((float)fr, &s_, &c_);
+                sn = s_;
+                cs = c_;
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) vis[k] = cmul(vis[k], cpx<T>{cs, sn});
+        }
+        if (a.transform) {  // baseline_transform_policies.h:145-148
+            u = u - w * g.u_term;
+            v = v - w * g.v_term;
+        }
+        u *= a.u_scale;
+        v *= a.v_scale;
+        // AA tap choice (convolution_policies.h:88-96 under FE_DOWNWARD)
+        const double flu = floor(u), flv = floor(v);
+        const int x0 = (int)flu + (int)a.off_x;
+        const int y0 = (int)flv + (int)a.off_y;
+        const int fu = (int)((1.0 - u + flu) * a.OS);
+        const int fv = (int)((1.0 - v + flv) * a.OS);
+        if (x0 < 0 || y0 < 0 || x0 + W + 2 >= (int)a.nx || y0 + W + 2 >= (int)a.ny) return false;
+        r.x0 = x0;
+        r.y0 = y0;
+        r.cgi = (int)cgi;
+        const int xm = x0 % W, ym = y0 % W;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            r.cu[j] = sconv[fu + mod_w(j - xm) * (int)a.OS];
+            r.cv[j] = sconv[fv + mod_w(j - ym) * (int)a.OS];
+        }
+        const double ts = a.tapsum[fu] * a.tapsum[fv];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) nw[k] = (double)wt[k] * ts;
+        return true;
+    }
+
+    static __device__ __forceinline__ void flush1(const GridArgs &a, uint32_t facet, int cgi,
+                                                  int x, int y, int c0,
+                                                  acc1<T, KAHAN> (&acc)[CPL][2]) {
+        bool nz = false;
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) nz |= (acc[k][0].s != T(0)) | (acc[k][1].s != T(0));
+        if (nz) {
+            double *cell = a.grid + (((((size_t)facet * a.cube + cgi) * a.ny + y) * a.nx + x) *
+                                         (size_t)a.pstride + c0) * 2;
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                unsafeAtomicAdd(cell + 2 * k, acc[k][0].value());
+                unsafeAtomicAdd(cell + 2 * k + 1, acc[k][1].value());
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            acc[k][0].zero();
+            acc[k][1].zero();
+        }
+    }
+
+    static __device__ void run(const GridArgs &a, const T *sconv, R_t *srec) {
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int gw = lane / G;
+        const int gl = lane - gw * G;
+        if (gw >= GPW) return;
+        const uint64_t stream =
+            ((uint64_t)blockIdx.x * (BLOCK / 64) + wave) * (uint64_t)GPW + (uint64_t)gw;
+        if (stream >= a.n_streams) return;
+        uint64_t t = stream;
+        const uint32_t facet = (uint32_t)(t % a.facets);
+        t /= a.facets;
+        const uint32_t seg = (uint32_t)(t % a.n_segments);
+        t /= a.n_segments;
+        const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
+        const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
+        const uint32_t b0 = a.bl_start[bl];
+        const uint32_t nb = a.bl_start[bl + 1] - b0;
+        const uint32_t r_lo = (uint32_t)(((uint64_t)nb * seg) / a.n_segments);
+        const uint32_t r_hi = (uint32_t)(((uint64_t)nb * (seg + 1)) / a.n_segments);
+        const uint32_t c_lo = cb * a.chan_per_stream;
+        const uint32_t c_hi = min(a.chan, c_lo + a.chan_per_stream);
+        const uint32_t ncb = c_hi - c_lo;
+        const uint32_t nsamp = (r_hi - r_lo) * ncb;
+        const FacetGeom g = a.geom[facet];
+
+        const int ua = gl % W;          // u residue of this lane
+        const int c0 = (gl / W) * CPL;  // first correlation of this lane
+        R_t *grp = srec + wave * 64 + gw * G;
+
+        acc1<T, KAHAN> acc[W][CPL][2];
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                acc[r][k][0].zero();
+                acc[r][k][1].zero();
+            }
+        int px0 = -1, py0 = -1, pcgi = -1;   // footprint of the accumulated samples
+        int ncgi = -1;
+        double nacc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) nacc[k] = 0;
+        unsigned long long cnt = 0;
+
+        for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
+            const uint32_t s = step0 + gl;
+            R_t rec;
+            rec.cgi = -1;
+            if (s < nsamp) {
+                const uint32_t tr = s / ncb;
+                const uint32_t j = s - tr * ncb;
+                const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
+                const uint32_t row = a.perm[b0 + r_lo + tr];
+                double nw[NC];
+                if (prep(a, g, sconv, facet, row, c, rec, nw)) {
+                    ++cnt;
+                    if constexpr (MODE != MODE_PSF) {
+                        if (rec.cgi != ncgi) {
+                            if (ncgi >= 0) {
+#pragma unroll
+                                for (int k = 0; k < NC; ++k) {
+                                    unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
+                                                    nacc[k]);
+                                    nacc[k] = 0;
+                                }
+                            }
+                            ncgi = rec.cgi;
+                        }
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) nacc[k] += nw[k];
+                    }
+                } else {
+                    rec.cgi = -1;
+                }
+            }
+            grp[gl] = rec;
+            wave_sync();
+            const int kmax = (int)min((uint32_t)G, nsamp - step0);
+            for (int kk = 0; kk < kmax; ++kk) {
+                const int cgi = grp[kk].cgi;
+                if (cgi < 0) continue;
+                const int x0 = grp[kk].x0, y0 = grp[kk].y0;
+                if (x0 != px0 || y0 != py0 || cgi != pcgi) {
+                    // footprint moved: flush the points this lane no longer owns
+                    if (pcgi >= 0) {
+                        const int xo = px0 + mod_w(ua - px0 % W);
+                        const int xn = x0 + mod_w(ua - x0 % W);
+                        const bool all = (cgi != pcgi) || (xn != xo);
+                        const int pym = py0 % W, ym = y0 % W;
+#pragma unroll
+                        for (int r = 0; r < W; ++r) {
+                            const int yo = py0 + mod_w(r - pym);
+                            const int yn = y0 + mod_w(r - ym);
+                            if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+                        }
+                    }
+                    px0 = x0;
+                    py0 = y0;
+                    pcgi = cgi;
+                }
+                const T cu = grp[kk].cu[ua];
+                cpx<T> vv[CPL];
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) vv[k] = grp[kk].vis[c0 + k];
+#pragma unroll
+                for (int r = 0; r < W; ++r) {
+                    const T wgt = cu * grp[kk].cv[r];
+#pragma unroll
+                    for (int k = 0; k < CPL; ++k) {
+                        acc[r][k][0].add(vv[k].re * wgt);
+                        acc[r][k][1].add(vv[k].im * wgt);
+                    }
+                }
+            }
+            wave_sync();
+        }
+        if (pcgi >= 0) {
+            const int xo = px0 + mod_w(ua - px0 % W);
+            const int pym = py0 % W;
+#pragma unroll
+            for (int r = 0; r < W; ++r) flush1(a, facet, pcgi, xo, py0 + mod_w(r - pym), c0, acc[r]);
+        }
+        // each lane publishes the normalisation sums and counts of the samples it prepared
+        if constexpr (MODE != MODE_PSF) {
+            if (ncgi >= 0) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                    unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k, nacc[k]);
+            }
+        }
+        if (cnt) {
+            atomicAdd(a.counters, cnt);
+            atomicAdd(a.counters + 1, cnt * (unsigned long long)(W * W));
+        }
+    }
+};
+
+template <int MODE, int PATH, int W>
+__global__ __launch_bounds__(BLOCK) void romein_w_kernel(GridArgs a) {
+    typedef GridderW<MODE, PATH, W> Gd;
+    typedef typename Gd::T T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *sconv = reinterpret_cast<T *>(smem);
+    const uint32_t conv_bytes = (a.csz * sizeof(T) + 15) & ~15u;
+    auto *srec = reinterpret_cast<typename Gd::R_t *>(smem + conv_bytes);
+    const T *gconv = reinterpret_cast<const T *>(a.conv);
+    for (uint32_t i = threadIdx.x; i < a.csz; i += BLOCK) sconv[i] = gconv[i];
+    __syncthreads();
+    Gd::run(a, sconv, srec);
+}
+
+template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &a, hipStream_t st) {
+    typedef GridderW<MODE, PATH, W> Gd;
+    const uint64_t waves = (a.n_streams + Gd::GPW - 1) / Gd::GPW;
+    const uint64_t blocks = (waves + BLOCK / 64 - 1) / (BLOCK / 64);
+    if (blocks == 0) return hipSuccess;
+    const size_t smem = ((a.csz * sizeof(typename Gd::T) + 15) & ~15u) +
+                        BLOCK * sizeof(typename Gd::R_t);
+    romein_w_kernel<MODE, PATH, W><<<dim3((uint32_t)blocks), dim3(BLOCK), smem, st>>>(a);
+    return hipGetLastError();
+}
+
+template <int MODE, int W> static hipError_t launch_p(const GridArgs &a, int path, hipStream_t st) {
+    switch (path) {
+        case PATH_FP64: return launch_t<MODE, PATH_FP64, W>(a, st);
+        case PATH_F32K: return launch_t<MODE, PATH_F32K, W>(a, st);
+        case PATH_F32: return launch_t<MODE, PATH_F32, W>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int W> static hipError_t launch_m(const GridArgs &a, int mode, int path, hipStream_t st) {
+    switch (mode) {
+        case MODE_1: return launch_p<MODE_1, W>(a, path, st);
+        case MODE_2: return launch_p<MODE_2, W>(a, path, st);
+        case MODE_4: return launch_p<MODE_4, W>(a, path, st);
+        case MODE_4J: return launch_p<MODE_4J, W>(a, path, st);
+        case MODE_PSF: return launch_p<MODE_PSF, W>(a, path, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace rw
+
+// Lane groups per wavefront for a support width, or 0 when no specialisation exists.
+int romein_w_groups_per_wave(int W, int mode) {
+    const int split = (mode == MODE_4 || mode == MODE_4J) ? 2 : 1;
+    switch (W) {
+        case 3: case 5: case 7: case 9: case 11: return 64 / (W * split);
+    }
+    return 0;
+}
+
+hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st) {
+    switch (a.W) {
+        case 3: return rw::launch_m<3>(a, mode, path, st);
+        case 5: return rw::launch_m<5>(a, mode, path, st);
+        case 7: return rw::launch_m<7>(a, mode, path, st);
+        case 9: return rw::launch_m<9>(a, mode, path, st);
+        case 11: return rw::launch_m<11>(a, mode, path, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace bam

@@ -18,6 +18,8 @@
 
 namespace bam {
 hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
+hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st);
+int romein_w_groups_per_wave(int W, int mode);
 int romein_max_width();
 size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
 hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
@@ -240,7 +242,7 @@ static int publish_results(gridding_parameters *p, bool psf) {
 
 // Streams: channels walked together and time segments per baseline.
 static void choose_streams(const gridding_parameters *p, size_t F, size_t B, size_t rows, int W,
-                           bool same_slice, uint32_t &cps, uint32_t &nseg) {
+                           int mode, bool same_slice, uint32_t &cps, uint32_t &nseg) {
     const size_t C = p->channel_count;
     cps = g.opts.channels_per_stream ? g.opts.channels_per_stream
                                      : (uint32_t)(same_slice ? std::min<size_t>(C, 16) : 1);
@@ -251,8 +253,9 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
         nseg = (uint32_t)std::max<size_t>(1, per_bl / g.opts.rows_per_stream);
     } else {
         // enough lane groups to fill 256 CUs x 4 SIMDs several waves deep
-        const int G = W * ((W + 7) / 8);
-        const size_t want_groups = (size_t)256 * 4 * 4 * (64 / std::max(G, 1));
+        int gpw = romein_w_groups_per_wave(W, mode);
+        if (gpw == 0) gpw = 64 / std::max(1, W * ((W + 7) / 8));
+        const size_t want_groups = (size_t)256 * 4 * 4 * gpw;
         const size_t have = F * B * nblocks;
         nseg = 1;
         const size_t per_bl = std::max<size_t>(1, rows / std::max<size_t>(1, B));
@@ -398,7 +401,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
         same_slice = g.cube == 1 && (!psf || g.sfC == 1);
     }
     uint32_t cps, nseg;
-    choose_streams(p, g.F, B, rows, W, same_slice, cps, nseg);
+    choose_streams(p, g.F, B, rows, W, mode, same_slice, cps, nseg);
 
     a.perm = g.perm.as<uint32_t>();
     a.bl_start = g.blstart.as<uint32_t>();
@@ -444,7 +447,10 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.transform = facet ? 1 : 0;
 
     HIP_TRY(hipEventRecord(tc.k0, g.stream));
-    HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
+    if (romein_w_groups_per_wave(W, mode) > 0 && !g.opts.reserved[0])
+        HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
+    else
+        HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
     HIP_TRY(hipEventRecord(tc.k1, g.stream));
     tc.has_kernel = true;
     HIP_TRY(hipEventRecord(tc.stop, g.stream));

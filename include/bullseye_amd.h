@@ -199,7 +199,7 @@ typedef struct bullseye_options {
                                    taper (DFT of the sampled AA kernel) */
     uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto) */
     uint32_t rows_per_stream;   /* Romein stream: time rows walked per work item (0 = auto) */
-    uint32_t reserved[10];
+    uint32_t reserved[10];      /* reserved[0] = 1 forces the generic (any-support) kernel */
 } bullseye_options;
 
 int bullseye_set_options(const bullseye_options *options);
