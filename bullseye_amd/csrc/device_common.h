@@ -51,6 +51,7 @@ struct GridArgs {
     const uint32_t *ant1, *ant2;               // [row]
     const uint64_t *time_ids;                  // [row]
     const visibility_base_type *jones;         // complex [t][ant][facet][spw][chan][4]
+    const void *jones_inv;                     // the same, inverted, in the path's type (acc_t)
     const uint8_t *enabled;                    // [spw*chan]
     const uint64_t *cgi;                       // [spw*chan] grid slice per channel
     const uint32_t *perm;                      // [row] rows sorted by baseline, time order kept
@@ -98,6 +99,12 @@ template <typename T, bool KAHAN> struct acc1 {
         } else {
             s += x;
         }
+    }
+    // Kahan step with a pre-formed term y = x - c (the compensation already subtracted).
+    __device__ __forceinline__ void add_pre(T y) {
+        T t = s + y;
+        c = (t - s) - y;
+        s = t;
     }
     __device__ __forceinline__ double value() const {
         return KAHAN ? (double)s - (double)c : (double)s;

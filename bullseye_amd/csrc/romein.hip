@@ -35,23 +35,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T> __device__ __forceinline__ cpx<T> cinv(cpx<T> d) {
-    T den = d.re * d.re + d.im * d.im;
-    return {d.re / den, -d.im / den};
-}
-
-// 2x2 complex inverse (jones_2x2.h:88-106); flags a singular matrix.
-template <typename T>
-__device__ __forceinline__ void jinv(const cpx<T> m[4], cpx<T> o[4], int *err) {
-    cpx<T> det = {m[0].re * m[3].re - m[0].im * m[3].im - (m[1].re * m[2].re - m[1].im * m[2].im),
-                  m[0].re * m[3].im + m[0].im * m[3].re - (m[1].re * m[2].im + m[1].im * m[2].re)};
-    if (det.re == 0 && det.im == 0) atomicOr(err, 1);
-    cpx<T> di = cinv(det);
-    o[0] = cmul(m[3], di);
-    o[1] = cmul(cpx<T>{-m[1].re, -m[1].im}, di);
-    o[2] = cmul(cpx<T>{-m[2].re, -m[2].im}, di);
-    o[3] = cmul(m[0], di);
-}
 
 template <int MODE, int PATH, bool WPROJ> struct Gridder {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
@@ -107,14 +90,8 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
                                    a.spw_count + s) * a.chan + c;
             const size_t j2 = ((((size_t)t * a.antenna_count + a.ant2[row]) * a.facets + facet) *
                                    a.spw_count + s) * a.chan + c;
-            cpx<T> P[4], Q[4], Pi[4], Qi[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                P[k] = {(T)a.jones[8 * j1 + 2 * k], (T)a.jones[8 * j1 + 2 * k + 1]};
-                Q[k] = {(T)a.jones[8 * j2 + 2 * k], (T)a.jones[8 * j2 + 2 * k + 1]};
-            }
-            jinv(P, Pi, a.error_flag);
-            jinv(Q, Qi, a.error_flag);
+            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j1;
+            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j2;
             const cpx<T> h0 = conjg(Qi[0]), h1 = conjg(Qi[2]), h2 = conjg(Qi[1]), h3 = conjg(Qi[3]);
             const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
             const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));

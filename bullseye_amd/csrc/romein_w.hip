@@ -1,15 +1,20 @@
 // romein_w.hip -- Romein gridding specialised on the kernel's full support W (3..11).
 //
-// Same decomposition and semantics as romein.hip, with three changes for the common case:
+// Same decomposition and semantics as romein.hip, with these changes for the common case:
 //  * W is a compile-time constant, so residues and ownership are constant-folded;
 //  * the lane that prepares a sample also writes the sample's taps *rotated to residue
 //    order* into its LDS record (cu[a] is the u-tap for the lane that owns x = a mod W, cv[j]
 //    the v-tap for the accumulator that owns y = j mod W), so the walk is loads + MACs;
 //  * 4-correlation modes split the correlations over two lanes (CPL = 2 per lane), which
-//    halves the accumulator registers per lane.
+//    halves the accumulator registers per lane;
+//  * FP32 compensated sums take one Kahan step per run of QUAD samples on an unchanged
+//    footprint: the QUAD products are chained by FMA onto -c and the result is added with
+//    the Kahan correction (footprints move every few tens of samples along a baseline, so
+//    nearly every run is a full QUAD).  The error stays O(u) per term, independent of the
+//    number of samples a register sums.
 // A lane group is W * NSPLIT lanes: lane (h, a) owns u-residue a and correlations
-// [h*CPL, (h+1)*CPL) of all W v-residues.  When a sample leaves the footprint where it was
-// (x0, y0, grid slice) unchanged, no ownership check is needed (fast path).
+// [h*CPL, (h+1)*CPL) of all W v-residues.  Each group's LDS records start on a different
+// LDS bank so the four groups of a wavefront read without conflicts.
 #include "device_common.h"
 
 namespace bam {
@@ -17,28 +22,12 @@ namespace bam {
 namespace rw {
 
 constexpr int BLOCK = 256;
+constexpr int QUAD = 4;
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename T> __device__ __forceinline__ cpx<T> cinv(cpx<T> d) {
-    T den = d.re * d.re + d.im * d.im;
-    return {d.re / den, -d.im / den};
-}
-
-template <typename T>
-__device__ __forceinline__ void jinv(const cpx<T> m[4], cpx<T> o[4], int *err) {
-    cpx<T> det = {m[0].re * m[3].re - m[0].im * m[3].im - (m[1].re * m[2].re - m[1].im * m[2].im),
-                  m[0].re * m[3].im + m[0].im * m[3].re - (m[1].re * m[2].im + m[1].im * m[2].re)};
-    if (det.re == 0 && det.im == 0) atomicOr(err, 1);
-    cpx<T> di = cinv(det);
-    o[0] = cmul(m[3], di);
-    o[1] = cmul(cpx<T>{-m[1].re, -m[1].im}, di);
-    o[2] = cmul(cpx<T>{-m[2].re, -m[2].im}, di);
-    o[3] = cmul(m[0], di);
 }
 
 template <typename T, int NC, int W> struct Rec {
@@ -57,6 +46,11 @@ template <int MODE, int PATH, int W> struct GridderW {
     typedef typename PathTraits<PATH>::acc_t T;
     static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
     typedef Rec<T, NC, W> R_t;
+    // bytes between the record arrays of consecutive groups: 16-byte aligned and starting
+    // 8 banks (of 4 bytes, 32 banks) after the previous group
+    static constexpr int GRAW = (G * (int)sizeof(R_t) + 15) & ~15;
+    static constexpr int GSTRIDE = GRAW + ((8 - (GRAW / 4) % 32 + 32) % 32) * 4;
+    static constexpr int WAVE_BYTES = GPW * GSTRIDE;
 
     static __device__ __forceinline__ int mod_w(int v) {  // v in (-W, 2W)
         v = v < 0 ? v + W : v;
@@ -108,23 +102,18 @@ template <int MODE, int PATH, int W> struct GridderW {
                                    a.spw_count + s) * a.chan + c;
             const size_t j2 = ((((size_t)t * a.antenna_count + a.ant2[row]) * a.facets + facet) *
                                    a.spw_count + s) * a.chan + c;
-            cpx<T> P[4], Q[4], Pi[4], Qi[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                P[k] = {(T)a.jones[8 * j1 + 2 * k], (T)a.jones[8 * j1 + 2 * k + 1]};
-                Q[k] = {(T)a.jones[8 * j2 + 2 * k], (T)a.jones[8 * j2 + 2 * k + 1]};
-            }
-            jinv(P, Pi, a.error_flag);
-            jinv(Q, Qi, a.error_flag);
+            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j1;
+            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j2;
             const cpx<T> h0 = conjg(Qi[0]), h1 = conjg(Qi[2]), h2 = conjg(Qi[1]), h3 = conjg(Qi[3]);
             const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
             const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));
             const cpx<T> t2 = cadd(cmul(vis[2], h0), cmul(vis[3], h2));
             const cpx<T> t3 = cadd(cmul(vis[2], h1), cmul(vis[3], h3));
-            vis[0] = cadd(cmul(Pi[0], t0), cmul(Pi[1], t2));
-            vis[1] = cadd(cmul(Pi[0], t1), cmul(Pi[1], t3));
-            vis[2] = cadd(cmul(Pi[2], t0), cmul(Pi[3], t2));
-            vis[3] = cadd(cmul(Pi[2], t1), cmul(Pi[3], t3));
+            const cpx<T> p0 = Pi[0], p1 = Pi[1], p2 = Pi[2], p3 = Pi[3];
+            vis[0] = cadd(cmul(p0, t0), cmul(p1, t2));
+            vis[1] = cadd(cmul(p0, t1), cmul(p1, t3));
+            vis[2] = cadd(cmul(p2, t0), cmul(p3, t2));
+            vis[3] = cadd(cmul(p2, t1), cmul(p3, t3));
         }
         bool any = false;
 #pragma unroll
@@ -182,6 +171,8 @@ template <int MODE, int PATH, int W> struct GridderW {
         return true;
     }
 
+    typedef acc1<T, KAHAN> Acc[W][CPL][2];
+
     static __device__ __forceinline__ void flush1(const GridArgs &a, uint32_t facet, int cgi,
                                                   int x, int y, int c0,
                                                   acc1<T, KAHAN> (&acc)[CPL][2]) {
@@ -204,14 +195,101 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
     }
 
-    static __device__ void run(const GridArgs &a, const T *sconv, R_t *srec) {
+    // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.
+    static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int ua,
+                                                   int c0, int x0, int y0, int cgi, int &px0,
+                                                   int &py0, int &pcgi, Acc &acc) {
+        if (x0 == px0 && y0 == py0 && cgi == pcgi) return;
+        if (pcgi >= 0) {
+            const int xo = px0 + mod_w(ua - px0 % W);
+            const int xn = x0 + mod_w(ua - x0 % W);
+            const bool all = (cgi != pcgi) || (xn != xo);
+            const int pym = py0 % W, ym = y0 % W;
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const int yo = py0 + mod_w(r - pym);
+                const int yn = y0 + mod_w(r - ym);
+                if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+            }
+        }
+        px0 = x0;
+        py0 = y0;
+        pcgi = cgi;
+    }
+
+    // One sample into the register sums (classic Kahan step on the compensated path).
+    static __device__ __forceinline__ void add_one(const R_t &rec, int ua, int c0, Acc &acc) {
+        const T cu = rec.cu[ua];
+        cpx<T> vs[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            vs[k] = rec.vis[c0 + k];
+            vs[k].re *= cu;
+            vs[k].im *= cu;
+        }
+#pragma unroll
+        for (int r = 0; r < W; ++r) {
+            const T cv = rec.cv[r];
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                if constexpr (KAHAN) {
+                    acc[r][k][0].add_pre(vs[k].re * cv - acc[r][k][0].c);
+                    acc[r][k][1].add_pre(vs[k].im * cv - acc[r][k][1].c);
+                } else {
+                    acc[r][k][0].s = fma(vs[k].re, cv, acc[r][k][0].s);
+                    acc[r][k][1].s = fma(vs[k].im, cv, acc[r][k][1].s);
+                }
+            }
+        }
+    }
+
+    // QUAD samples on one footprint into the compensated sums with a single Kahan step.
+    static __device__ __forceinline__ void add_quad(const R_t *recs, int ua, int c0, Acc &acc) {
+        T y[W][CPL][2];
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                y[r][k][0] = -acc[r][k][0].c;
+                y[r][k][1] = -acc[r][k][1].c;
+            }
+#pragma unroll
+        for (int q = 0; q < QUAD; ++q) {
+            const R_t &rec = recs[q];
+            const T cu = rec.cu[ua];
+            cpx<T> vs[CPL];
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                vs[k] = rec.vis[c0 + k];
+                vs[k].re *= cu;
+                vs[k].im *= cu;
+            }
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const T cv = rec.cv[r];
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) {
+                    y[r][k][0] = fma(vs[k].re, cv, y[r][k][0]);
+                    y[r][k][1] = fma(vs[k].im, cv, y[r][k][1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                acc[r][k][0].add_pre(y[r][k][0]);
+                acc[r][k][1].add_pre(y[r][k][1]);
+            }
+    }
+
+    static __device__ void run(const GridArgs &a, const T *sconv, unsigned char *swave) {
         const int lane = threadIdx.x & 63;
-        const int wave = threadIdx.x >> 6;
         const int gw = lane / G;
         const int gl = lane - gw * G;
         if (gw >= GPW) return;
         const uint64_t stream =
-            ((uint64_t)blockIdx.x * (BLOCK / 64) + wave) * (uint64_t)GPW + (uint64_t)gw;
+            ((uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
         if (stream >= a.n_streams) return;
         uint64_t t = stream;
         const uint32_t facet = (uint32_t)(t % a.facets);
@@ -232,9 +310,9 @@ template <int MODE, int PATH, int W> struct GridderW {
 
         const int ua = gl % W;          // u residue of this lane
         const int c0 = (gl / W) * CPL;  // first correlation of this lane
-        R_t *grp = srec + wave * 64 + gw * G;
+        R_t *grp = reinterpret_cast<R_t *>(swave + gw * GSTRIDE);
 
-        acc1<T, KAHAN> acc[W][CPL][2];
+        Acc acc;
 #pragma unroll
         for (int r = 0; r < W; ++r)
 #pragma unroll
@@ -251,73 +329,60 @@ template <int MODE, int PATH, int W> struct GridderW {
 
         for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
             const uint32_t s = step0 + gl;
-            R_t rec;
-            rec.cgi = -1;
-            if (s < nsamp) {
-                const uint32_t tr = s / ncb;
-                const uint32_t j = s - tr * ncb;
-                const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
-                const uint32_t row = a.perm[b0 + r_lo + tr];
-                double nw[NC];
-                if (prep(a, g, sconv, facet, row, c, rec, nw)) {
-                    ++cnt;
-                    if constexpr (MODE != MODE_PSF) {
-                        if (rec.cgi != ncgi) {
-                            if (ncgi >= 0) {
+            {
+                R_t rec;
+                rec.cgi = -1;
+                if (s < nsamp) {
+                    const uint32_t tr = s / ncb;
+                    const uint32_t j = s - tr * ncb;
+                    const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
+                    const uint32_t row = a.perm[b0 + r_lo + tr];
+                    double nw[NC];
+                    if (prep(a, g, sconv, facet, row, c, rec, nw)) {
+                        ++cnt;
+                        if constexpr (MODE != MODE_PSF) {
+                            if (rec.cgi != ncgi) {
+                                if (ncgi >= 0) {
 #pragma unroll
-                                for (int k = 0; k < NC; ++k) {
-                                    unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
-                                                    nacc[k]);
-                                    nacc[k] = 0;
+                                    for (int k = 0; k < NC; ++k) {
+                                        unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
+                                                        nacc[k]);
+                                        nacc[k] = 0;
+                                    }
                                 }
+                                ncgi = rec.cgi;
                             }
-                            ncgi = rec.cgi;
-                        }
 #pragma unroll
-                        for (int k = 0; k < NC; ++k) nacc[k] += nw[k];
+                            for (int k = 0; k < NC; ++k) nacc[k] += nw[k];
+                        }
+                    } else {
+                        rec.cgi = -1;
                     }
-                } else {
-                    rec.cgi = -1;
                 }
+                grp[gl] = rec;
             }
-            grp[gl] = rec;
             wave_sync();
             const int kmax = (int)min((uint32_t)G, nsamp - step0);
-            for (int kk = 0; kk < kmax; ++kk) {
-                const int cgi = grp[kk].cgi;
-                if (cgi < 0) continue;
-                const int x0 = grp[kk].x0, y0 = grp[kk].y0;
-                if (x0 != px0 || y0 != py0 || cgi != pcgi) {
-                    // footprint moved: flush the points this lane no longer owns
-                    if (pcgi >= 0) {
-                        const int xo = px0 + mod_w(ua - px0 % W);
-                        const int xn = x0 + mod_w(ua - x0 % W);
-                        const bool all = (cgi != pcgi) || (xn != xo);
-                        const int pym = py0 % W, ym = y0 % W;
+            int kk = 0;
+            if constexpr (KAHAN) {
+                while (kk + QUAD <= kmax) {
+                    const int cg = grp[kk].cgi, x0 = grp[kk].x0, y0 = grp[kk].y0;
+                    bool same = cg >= 0;
 #pragma unroll
-                        for (int r = 0; r < W; ++r) {
-                            const int yo = py0 + mod_w(r - pym);
-                            const int yn = y0 + mod_w(r - ym);
-                            if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
-                        }
-                    }
-                    px0 = x0;
-                    py0 = y0;
-                    pcgi = cgi;
+                    for (int q = 1; q < QUAD; ++q)
+                        same = same && grp[kk + q].cgi == cg && grp[kk + q].x0 == x0 &&
+                               grp[kk + q].y0 == y0;
+                    if (!same) break;
+                    move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
+                    add_quad(grp + kk, ua, c0, acc);
+                    kk += QUAD;
                 }
-                const T cu = grp[kk].cu[ua];
-                cpx<T> vv[CPL];
-#pragma unroll
-                for (int k = 0; k < CPL; ++k) vv[k] = grp[kk].vis[c0 + k];
-#pragma unroll
-                for (int r = 0; r < W; ++r) {
-                    const T wgt = cu * grp[kk].cv[r];
-#pragma unroll
-                    for (int k = 0; k < CPL; ++k) {
-                        acc[r][k][0].add(vv[k].re * wgt);
-                        acc[r][k][1].add(vv[k].im * wgt);
-                    }
-                }
+            }
+            for (; kk < kmax; ++kk) {
+                const int cg = grp[kk].cgi;
+                if (cg < 0) continue;
+                move_to(a, facet, ua, c0, grp[kk].x0, grp[kk].y0, cg, px0, py0, pcgi, acc);
+                add_one(grp[kk], ua, c0, acc);
             }
             wave_sync();
         }
@@ -349,11 +414,10 @@ __global__ __launch_bounds__(BLOCK) void romein_w_kernel(GridArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     T *sconv = reinterpret_cast<T *>(smem);
     const uint32_t conv_bytes = (a.csz * sizeof(T) + 15) & ~15u;
-    auto *srec = reinterpret_cast<typename Gd::R_t *>(smem + conv_bytes);
     const T *gconv = reinterpret_cast<const T *>(a.conv);
     for (uint32_t i = threadIdx.x; i < a.csz; i += BLOCK) sconv[i] = gconv[i];
     __syncthreads();
-    Gd::run(a, sconv, srec);
+    Gd::run(a, sconv, smem + conv_bytes + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
 template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &a, hipStream_t st) {
@@ -362,7 +426,7 @@ template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &
     const uint64_t blocks = (waves + BLOCK / 64 - 1) / (BLOCK / 64);
     if (blocks == 0) return hipSuccess;
     const size_t smem = ((a.csz * sizeof(typename Gd::T) + 15) & ~15u) +
-                        BLOCK * sizeof(typename Gd::R_t);
+                        (BLOCK / 64) * (size_t)Gd::WAVE_BYTES;
     romein_w_kernel<MODE, PATH, W><<<dim3((uint32_t)blocks), dim3(BLOCK), smem, st>>>(a);
     return hipGetLastError();
 }

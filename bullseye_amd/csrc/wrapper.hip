@@ -20,6 +20,8 @@ namespace bam {
 hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
 hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st);
 int romein_w_groups_per_wave(int W, int mode);
+hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, bool fp64,
+                        int *err, hipStream_t st);
 int romein_max_width();
 size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
 hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
@@ -83,7 +85,7 @@ struct State {
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
-    DevBuf out_tmp, fft_scratch, fft_real;
+    DevBuf out_tmp, fft_scratch, fft_real, jinv;
     std::vector<normalization_base_type> host_norm;
     std::vector<double> taper_x, taper_y;
     std::vector<FacetGeom> geo_host;  // kept alive for the asynchronous upload
@@ -343,6 +345,11 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
         HIP_TRY(stage(g.in_jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d));
         a.jones = (const visibility_base_type *)d;
         if (!a.time_ids) return fail("timestamp_ids is NULL");
+        // jones_2x2.h:113-126 invert_all, into a library buffer in the compute precision
+        const bool fp64 = g.path == PATH_FP64;
+        HIP_TRY(g.jinv.ensure(nj * 8 * (fp64 ? 8 : 4)));
+        HIP_TRY(invert_jones(a.jones, g.jinv.p, nj, fp64, g.errflag.as<int>(), g.stream));
+        a.jones_inv = g.jinv.p;
     }
 
     // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre)
@@ -629,7 +636,7 @@ int releaseLibrary(void) {
                       &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
-                      &g.fft_scratch, &g.fft_real};
+                      &g.fft_scratch, &g.fft_real, &g.jinv};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
