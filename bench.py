@@ -161,7 +161,7 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle
-        n = args.cpu_sample_rows or (obs.rows // 20)
+        n = args.cpu_sample_rows or (obs.rows // 4)
         sub = (0, n)
         spec64 = ImagingSpec(**{**spec.__dict__, "jones": jones_reference_layout(
             obs, im["facets"], np.complex128)})

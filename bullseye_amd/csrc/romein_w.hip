@@ -5,8 +5,8 @@
 //  * the lane that prepares a sample also writes the sample's taps *rotated to residue
 //    order* into its LDS record (cu[a] is the u-tap for the lane that owns x = a mod W, cv[j]
 //    the v-tap for the accumulator that owns y = j mod W), so the walk is loads + MACs;
-//  * 4-correlation modes split the correlations over two lanes (CPL = 2 per lane), which
-//    halves the accumulator registers per lane;
+//  * 4-correlation modes split the correlations over lanes (CPL = 1 per lane on the FP32
+//    paths, 2 on the FP64 path), which cuts the accumulator registers per lane;
 //  * FP32 compensated sums take one Kahan step per run of QUAD samples on an unchanged
 //    footprint: the QUAD products are chained by FMA onto -c and the result is added with
 //    the Kahan correction (footprints move every few tens of samples along a baseline, so
@@ -39,7 +39,9 @@ template <typename T, int NC, int W> struct Rec {
 
 template <int MODE, int PATH, int W> struct GridderW {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
-    static constexpr int CPL = NC == 4 ? 2 : NC;      // correlations per lane
+    // correlations per lane: one on the FP32 paths (fewest registers), two on the FP64 path
+    // (its double-width records make per-sample LDS reads the larger cost)
+    static constexpr int CPL = NC == 4 ? (PATH == PATH_FP64 ? 2 : 1) : NC;
     static constexpr int NSPLIT = NC / CPL;
     static constexpr int G = W * NSPLIT;               // lanes per group
     static constexpr int GPW = 64 / G;                 // groups per wavefront
@@ -58,7 +60,7 @@ template <int MODE, int PATH, int W> struct GridderW {
     }
 
     // Prepare one sample; returns false when it contributes nothing.
-    static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom &g,
+    static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom *__restrict__ gp,
                                                 const T *sconv, uint32_t facet, uint32_t row,
                                                 uint32_t c, R_t &r, double nw[NC]) {
         const uint32_t s = a.spw[row];
@@ -124,7 +126,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
         if (!any) return false;
         if (a.phase) {  // phase_transform_policies.h:138-146, argument in turns, FP64
-            const double turns = u * g.dl + v * g.dm + w * g.dn;
+            const double turns = u * gp->dl + v * gp->dm + w * gp->dn;
             const double fr = 2.0 * (turns - rint(turns));
             T sn, cs;
             if constexpr (PATH == PATH_FP64) {
@@ -142,8 +144,8 @@ template <int MODE, int PATH, int W> struct GridderW {
             for (int k = 0; k < NC; ++k) vis[k] = cmul(vis[k], cpx<T>{cs, sn});
         }
         if (a.transform) {  // baseline_transform_policies.h:145-148
-            u = u - w * g.u_term;
-            v = v - w * g.v_term;
+            u = u - w * gp->u_term;
+            v = v - w * gp->v_term;
         }
         u *= a.u_scale;
         v *= a.v_scale;
@@ -306,7 +308,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         const uint32_t c_hi = min(a.chan, c_lo + a.chan_per_stream);
         const uint32_t ncb = c_hi - c_lo;
         const uint32_t nsamp = (r_hi - r_lo) * ncb;
-        const FacetGeom g = a.geom[facet];
+        const FacetGeom *gp = a.geom + facet;
 
         const int ua = gl % W;          // u residue of this lane
         const int c0 = (gl / W) * CPL;  // first correlation of this lane
@@ -338,7 +340,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                     const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (prep(a, g, sconv, facet, row, c, rec, nw)) {
+                    if (prep(a, gp, sconv, facet, row, c, rec, nw)) {
                         ++cnt;
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
@@ -359,6 +361,16 @@ template <int MODE, int PATH, int W> struct GridderW {
                         rec.cgi = -1;
                     }
                 }
+                if (rec.cgi < 0) {
+                    // contributes nothing; zero taps so it can ride inside a QUAD run
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) rec.vis[k] = {T(0), T(0)};
+#pragma unroll
+                    for (int j = 0; j < W; ++j) {
+                        rec.cu[j] = T(0);
+                        rec.cv[j] = T(0);
+                    }
+                }
                 grp[gl] = rec;
             }
             wave_sync();
@@ -366,15 +378,28 @@ template <int MODE, int PATH, int W> struct GridderW {
             int kk = 0;
             if constexpr (KAHAN) {
                 while (kk + QUAD <= kmax) {
-                    const int cg = grp[kk].cgi, x0 = grp[kk].x0, y0 = grp[kk].y0;
-                    bool same = cg >= 0;
+                    // a run of QUAD records whose valid members share one footprint; invalid
+                    // records carry zero taps.  The footprint is that of the first valid one.
+                    int cg = -1, x0 = 0, y0 = 0;
+                    bool same = true;
 #pragma unroll
-                    for (int q = 1; q < QUAD; ++q)
-                        same = same && grp[kk + q].cgi == cg && grp[kk + q].x0 == x0 &&
-                               grp[kk + q].y0 == y0;
+                    for (int q = 0; q < QUAD; ++q) {
+                        const int c1 = grp[kk + q].cgi;
+                        if (c1 < 0) continue;
+                        const int x1 = grp[kk + q].x0, y1 = grp[kk + q].y0;
+                        if (cg < 0) {
+                            cg = c1;
+                            x0 = x1;
+                            y0 = y1;
+                        } else {
+                            same = same && c1 == cg && x1 == x0 && y1 == y0;
+                        }
+                    }
                     if (!same) break;
-                    move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
-                    add_quad(grp + kk, ua, c0, acc);
+                    if (cg >= 0) {
+                        move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
+                        add_quad(grp + kk, ua, c0, acc);
+                    }
                     kk += QUAD;
                 }
             }
@@ -454,8 +479,8 @@ template <int W> static hipError_t launch_m(const GridArgs &a, int mode, int pat
 }  // namespace rw
 
 // Lane groups per wavefront for a support width, or 0 when no specialisation exists.
-int romein_w_groups_per_wave(int W, int mode) {
-    const int split = (mode == MODE_4 || mode == MODE_4J) ? 2 : 1;
+int romein_w_groups_per_wave(int W, int mode, int path) {
+    const int split = (mode == MODE_4 || mode == MODE_4J) ? (path == PATH_FP64 ? 2 : 4) : 1;
     switch (W) {
         case 3: case 5: case 7: case 9: case 11: return 64 / (W * split);
     }

@@ -19,7 +19,7 @@
 namespace bam {
 hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
 hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st);
-int romein_w_groups_per_wave(int W, int mode);
+int romein_w_groups_per_wave(int W, int mode, int path);
 hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, bool fp64,
                         int *err, hipStream_t st);
 int romein_max_width();
@@ -255,7 +255,7 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
         nseg = (uint32_t)std::max<size_t>(1, per_bl / g.opts.rows_per_stream);
     } else {
         // enough lane groups to fill 256 CUs x 4 SIMDs several waves deep
-        int gpw = romein_w_groups_per_wave(W, mode);
+        int gpw = romein_w_groups_per_wave(W, mode, g.path);
         if (gpw == 0) gpw = 64 / std::max(1, W * ((W + 7) / 8));
         const size_t want_groups = (size_t)256 * 4 * 4 * gpw;
         const size_t have = F * B * nblocks;
@@ -454,7 +454,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.transform = facet ? 1 : 0;
 
     HIP_TRY(hipEventRecord(tc.k0, g.stream));
-    if (romein_w_groups_per_wave(W, mode) > 0 && !g.opts.reserved[0])
+    if (romein_w_groups_per_wave(W, mode, g.path) > 0 && !g.opts.reserved[0])
         HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
     else
         HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
