@@ -482,3 +482,28 @@ void oracle_ifftshift(grid_c *grid, size_t nx, size_t ny, size_t slices) {
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+/* offsetof every gridding_parameters field, in declaration order (ABI layout check) */
+#include <stddef.h>
+#define OFF(f) offsetof(gridding_parameters, f)
+size_t oracle_struct_offsets(size_t *out) {
+    const size_t o[] = {
+        OFF(visibilities), OFF(uvw_coords), OFF(reference_wavelengths), OFF(visibility_weights),
+        OFF(flags), OFF(flagged_rows), OFF(field_array), OFF(spw_index_array), OFF(imaging_field),
+        OFF(baseline_count), OFF(row_count), OFF(chunk_max_row_count), OFF(channel_count),
+        OFF(number_of_polarization_terms), OFF(number_of_polarization_terms_being_gridded),
+        OFF(spw_count), OFF(no_timestamps_read), OFF(nx), OFF(ny), OFF(cell_size_x),
+        OFF(cell_size_y), OFF(conv), OFF(conv_support), OFF(conv_oversample),
+        OFF(polarization_index), OFF(second_polarization_index), OFF(output_buffer),
+        OFF(phase_centre_ra), OFF(phase_centre_dec), OFF(facet_centres), OFF(num_facet_centres),
+        OFF(jones_terms), OFF(should_invert_jones_terms), OFF(antenna_1_ids), OFF(antenna_2_ids),
+        OFF(timestamp_ids), OFF(antenna_count), OFF(enabled_channels), OFF(channel_grid_indicies),
+        OFF(cube_channel_dim_size), OFF(should_grid_sampling_function),
+        OFF(sampling_function_buffer), OFF(sampling_function_channel_grid_indicies),
+        OFF(sampling_function_channel_count), OFF(is_final_data_chunk), OFF(wplanes),
+        OFF(wmax_est), OFF(baseline_starting_indexes), OFF(antenna_jones_starting_indexes),
+        OFF(jones_time_indicies_per_antenna), OFF(normalization_terms), sizeof(gridding_parameters)};
+    const size_t n = sizeof(o) / sizeof(o[0]);
+    if (out) memcpy(out, o, sizeof(o));
+    return n;
+}

@@ -88,7 +88,7 @@ def fftshift_inplace(grid, inverse=False, precision="double"):
     [..., ny, nx]."""
     lib = load(precision)
     dt = np.complex128 if precision == "double" else np.complex64
-    g = np.ascontiguousarray(grid, dtype=dt)
+    g = np.array(grid, dtype=dt, copy=True, order="C")
     ny, nx = g.shape[-2:]
     slices = g.size // (nx * ny)
     fn = lib.oracle_ifftshift if inverse else lib.oracle_fftshift

@@ -102,3 +102,32 @@ def rel_err(got, want):
     peak = np.abs(want).max()
     d = np.abs(got - want)
     return float(np.sqrt(np.mean(d ** 2)) / peak), float(d.max() / peak)
+
+
+def tiny_obs(uvw, vis, n_ant=4, wavelength=1.0, n_corr=1, ant=None):
+    """An observation of explicit rows: uvw [rows, 3] (metres), vis [rows, chan, corr]."""
+    uvw = np.asarray(uvw, dtype=np.float64).reshape(-1, 3)
+    rows = uvw.shape[0]
+    vis = np.asarray(vis, dtype=np.complex64).reshape(rows, -1, n_corr)
+    n_chan = vis.shape[1]
+    if ant is None:
+        a1, a2 = np.triu_indices(n_ant, k=1)
+        a1 = np.resize(a1, rows)
+        a2 = np.resize(a2, rows)
+    else:
+        a1, a2 = ant
+    obs = synth.Observation(
+        n_ant=n_ant, n_time=max(1, rows), n_chan=n_chan, n_corr=n_corr,
+        ant1=np.asarray(a1, np.uint32), ant2=np.asarray(a2, np.uint32),
+        time_idx=np.arange(rows, dtype=np.uint64), uvw=uvw,
+        freqs=np.full(n_chan, synth.C_LIGHT / wavelength), vis=vis,
+        weights=np.ones(vis.shape, np.float32), flags=np.zeros(vis.shape, bool),
+        flagged_rows=np.zeros(rows, bool), field=np.zeros(rows, np.uint32),
+        spw=np.zeros(rows, np.uint32), phase_centre=(162000.0, -108000.0),
+        sources=np.zeros((0, 3)))
+    obs.wl = np.full((1, n_chan), wavelength)
+    return obs
+
+
+def u_scale(npix, cell):
+    return npix * cell * float(np.float32(np.pi / (180.0 * 3600.0)))
