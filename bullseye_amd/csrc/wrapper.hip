@@ -22,6 +22,14 @@ hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st
 int romein_w_groups_per_wave(int W, int mode, int path);
 hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, bool fp64,
                         int *err, hipStream_t st);
+void wproj_shape(int M, uint32_t &block, uint32_t &lanes, uint32_t &spb, int &pairs);
+hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, uint32_t wplanes,
+                               float wmax_f, uint32_t nseg_max, uint8_t *cls, hipStream_t st);
+hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,
+                        const uint32_t *plane_half, const double *plane_norm,
+                        const uint32_t *stream_list, uint32_t n_list, uint32_t wplanes,
+                        uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
+                        hipStream_t st);
 int romein_max_width();
 size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
 hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
@@ -86,6 +94,13 @@ struct State {
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
     DevBuf out_tmp, fft_scratch, fft_real, jinv;
+    // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
+    DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
+    std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
+    std::vector<uint32_t> wplane_half_host;
+    uint32_t w_nf = 0, w_fbase = 0;
+    std::vector<uint8_t> wcls_host;
+    std::vector<uint32_t> wlist_host;
     std::vector<normalization_base_type> host_norm;
     std::vector<double> taper_x, taper_y;
     std::vector<FacetGeom> geo_host;  // kept alive for the asynchronous upload
@@ -268,10 +283,10 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
 static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     if (!g.initialized) return fail("initLibrary has not been called");
     const bool psf = mode == MODE_PSF;
-    if (p->wplanes > 1) return fail("w-projection (wplanes > 1) is served by the batched "
-                                    "w-projection entry, not this build's AA path");
+    const bool wproj = g.wplanes > 1;
+    if ((p->wplanes > 1) != wproj) return fail("wplanes differs from initLibrary");
     const int W = (int)(2 * p->conv_support + 1);
-    if (W > romein_max_width())
+    if (!wproj && W > romein_max_width())
         return fail("conv_support %zu too large for the AA gridder (max full support %d)",
                     (size_t)p->conv_support, romein_max_width());
     if (p->conv_support != g.S || p->conv_oversample != g.OS)
@@ -409,6 +424,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     }
     uint32_t cps, nseg;
     choose_streams(p, g.F, B, rows, W, mode, same_slice, cps, nseg);
+    if (wproj) cps = same_slice ? (uint32_t)std::min<size_t>(C, 8) : 1;
 
     a.perm = g.perm.as<uint32_t>();
     a.bl_start = g.blstart.as<uint32_t>();
@@ -453,11 +469,60 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.phase = phase ? 1 : 0;
     a.transform = facet ? 1 : 0;
 
-    HIP_TRY(hipEventRecord(tc.k0, g.stream));
-    if (romein_w_groups_per_wave(W, mode, g.path) > 0 && !g.opts.reserved[0])
-        HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
-    else
-        HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
+    if (wproj) {
+        // w-projection: streams are SEG-row segments bucketed by their support class
+        const uint32_t SEG = 32;
+        std::vector<uint32_t> bls(B + 1);
+        HIP_TRY(hipMemcpyAsync(bls.data(), g.blstart.p, (B + 1) * 4, hipMemcpyDeviceToHost,
+                               g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        uint32_t max_nb = 0;
+        for (size_t b = 0; b < B; ++b) max_nb = std::max(max_nb, bls[b + 1] - bls[b]);
+        const uint32_t nsegw = std::max<uint32_t>(1, (max_nb + SEG - 1) / SEG);
+        a.n_segments = nsegw;
+        const uint64_t ncls = (uint64_t)B * a.n_chan_blocks * nsegw;
+        if (ncls * g.F > 0xffffffffull) return fail("too many w-projection streams");
+        a.n_streams = ncls * g.F;
+        const float wmax_f = (float)p->wmax_est;
+        if (!(wmax_f > 0)) return fail("wmax_est must be > 0 for w-projection");
+        HIP_TRY(g.wcls.ensure(ncls));
+        HIP_TRY(launch_stream_class(a, g.wplane_half.as<uint32_t>(), (uint32_t)g.wplanes, wmax_f,
+                                    nsegw, g.wcls.as<uint8_t>(), g.stream));
+        g.wcls_host.resize(ncls);
+        HIP_TRY(hipMemcpyAsync(g.wcls_host.data(), g.wcls.p, ncls, hipMemcpyDeviceToHost,
+                               g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        // counting sort of the streams by class (every facet of a segment shares its class)
+        std::vector<uint64_t> count(256, 0), start(257, 0);
+        for (uint64_t i = 0; i < ncls; ++i) count[g.wcls_host[i]] += g.F;
+        for (int c2 = 0; c2 < 256; ++c2) start[c2 + 1] = start[c2] + (c2 ? count[c2] : 0);
+        g.wlist_host.resize(start[256]);
+        std::vector<uint64_t> fill(start.begin(), start.end() - 1);
+        for (uint64_t i = 0; i < ncls; ++i) {
+            const uint8_t c2 = g.wcls_host[i];
+            if (!c2) continue;
+            for (size_t f = 0; f < g.F; ++f) g.wlist_host[fill[c2]++] = (uint32_t)(i * g.F + f);
+        }
+        HIP_TRY(g.wlist.ensure(std::max<size_t>(4, g.wlist_host.size() * 4)));
+        HIP_TRY(hipMemcpyAsync(g.wlist.p, g.wlist_host.data(), g.wlist_host.size() * 4,
+                               hipMemcpyHostToDevice, g.stream));
+        const void *taps = g.path == PATH_FP64 ? g.wtaps_d.p : g.wtaps_f.p;
+        HIP_TRY(hipEventRecord(tc.k0, g.stream));
+        for (int M = 1; M < 256; M += 2) {
+            if (!count[M]) continue;
+            HIP_TRY(launch_wproj(a, taps, g.wplane_off.as<uint64_t>(), g.wplane_half.as<uint32_t>(),
+                                 g.wplane_norm.as<double>(), g.wlist.as<uint32_t>() + start[M],
+                                 (uint32_t)count[M], (uint32_t)g.wplanes, g.w_nf, g.w_fbase,
+                                 (uint32_t)M, wmax_f, mode, g.path, g.stream));
+        }
+        HIP_TRY(hipStreamSynchronize(g.stream));  // wlist_host stays valid until here
+    } else {
+        HIP_TRY(hipEventRecord(tc.k0, g.stream));
+        if (romein_w_groups_per_wave(W, mode, g.path) > 0 && !g.opts.reserved[0])
+            HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
+        else
+            HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
+    }
     HIP_TRY(hipEventRecord(tc.k1, g.stream));
     tc.has_kernel = true;
     HIP_TRY(hipEventRecord(tc.stop, g.stream));
@@ -507,6 +572,16 @@ int bullseye_get_options(bullseye_options *o) {
 int bullseye_get_stats(bullseye_stats *s) {
     if (!s) return fail("NULL stats");
     *s = g.stats;
+    return 0;
+}
+
+int bullseye_set_wplane_support(const uint32_t *half_support, uint32_t nplanes) {
+    if (g.initialized) return fail("bullseye_set_wplane_support must precede initLibrary");
+    if (!half_support || !nplanes) {
+        g.wplane_half_req.clear();
+        return 0;
+    }
+    g.wplane_half_req.assign(half_support, half_support + nplanes);
     return 0;
 }
 
@@ -620,6 +695,76 @@ int initLibrary(gridding_parameters *p) {
         HIP_TRY(hipMemcpy(g.taper.as<double>() + g.nx, g.taper_y.data(), g.ny * 8,
                           hipMemcpyHostToDevice));
     }
+    if (g.wplanes > 1) {
+        // w-projection taps: conv[plane][csz][csz] complex -> contiguous W_p x W_p blocks per
+        // (plane, frac_v, frac_u), fraction indexes fbase .. fbase + nf - 1 (nearest rounding
+        // gives fractions in [OS/2, 3 OS/2])
+        const size_t P = g.wplanes, S = g.S, OS = g.OS, csz = g.csz;
+        g.wplane_half_host.assign(P, (uint32_t)S);
+        if (!g.wplane_half_req.empty()) {
+            if (g.wplane_half_req.size() != P)
+                return fail("bullseye_set_wplane_support gave %zu planes, wplanes is %zu",
+                            g.wplane_half_req.size(), P);
+            for (size_t q = 0; q < P; ++q) {
+                if (g.wplane_half_req[q] > S) return fail("plane %zu support exceeds conv_support", q);
+                g.wplane_half_host[q] = g.wplane_half_req[q];
+            }
+        }
+        if (2 * S + 1 > 255) return fail("w-projection support too large");
+        g.w_fbase = (uint32_t)(OS / 2);
+        g.w_nf = (uint32_t)(OS + 2);
+        const size_t nf = g.w_nf;
+        std::vector<uint64_t> off(P + 1, 0);
+        for (size_t q = 0; q < P; ++q) {
+            const size_t wq = 2 * g.wplane_half_host[q] + 1;
+            off[q + 1] = off[q] + nf * nf * wq * wq;
+        }
+        const size_t nconv = P * csz * csz;
+        std::vector<convolution_base_type> ch(2 * nconv);
+        if (dev_mode())
+            HIP_TRY(hipMemcpy(ch.data(), p->conv, 2 * nconv * sizeof(convolution_base_type),
+                              hipMemcpyDeviceToHost));
+        else
+            memcpy(ch.data(), p->conv, 2 * nconv * sizeof(convolution_base_type));
+        std::vector<double> td(2 * off[P]);
+        std::vector<float> tf(2 * off[P]);
+        std::vector<double> nrm(P * nf * nf, 0.0);
+        for (size_t q = 0; q < P; ++q) {
+            const size_t sp = g.wplane_half_host[q], wq = 2 * sp + 1;
+            for (size_t fv = 0; fv < nf; ++fv)
+                for (size_t fu = 0; fu < nf; ++fu) {
+                    double sum = 0;
+                    for (size_t tv = 0; tv < wq; ++tv)
+                        for (size_t tu = 0; tu < wq; ++tu) {
+                            const size_t iv = g.w_fbase + fv + (S - sp + tv) * OS;
+                            const size_t iu = g.w_fbase + fu + (S - sp + tu) * OS;
+                            double re = 0, im = 0;
+                            if (iv < csz && iu < csz) {
+                                const size_t ci = (q * csz + iv) * csz + iu;
+                                re = ch[2 * ci];
+                                im = ch[2 * ci + 1];
+                            }
+                            const size_t ti = off[q] + ((fv * nf + fu) * wq + tv) * wq + tu;
+                            td[2 * ti] = re;
+                            td[2 * ti + 1] = im;
+                            tf[2 * ti] = (float)re;
+                            tf[2 * ti + 1] = (float)im;
+                            sum += re;
+                        }
+                    nrm[(q * nf + fv) * nf + fu] = sum;
+                }
+        }
+        HIP_TRY(g.wtaps_d.ensure(td.size() * 8));
+        HIP_TRY(g.wtaps_f.ensure(tf.size() * 4));
+        HIP_TRY(g.wplane_off.ensure(P * 8));
+        HIP_TRY(g.wplane_half.ensure(P * 4));
+        HIP_TRY(g.wplane_norm.ensure(nrm.size() * 8));
+        HIP_TRY(hipMemcpy(g.wtaps_d.p, td.data(), td.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.wtaps_f.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.wplane_off.p, off.data(), P * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.wplane_half.p, g.wplane_half_host.data(), P * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.wplane_norm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipStreamSynchronize(g.stream));
     g.grid_seconds = g.inv_seconds = 0;
     g.stats = bullseye_stats{};
@@ -636,7 +781,8 @@ int releaseLibrary(void) {
                       &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
-                      &g.fft_scratch, &g.fft_real, &g.jinv};
+                      &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
+                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;

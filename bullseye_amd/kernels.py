@@ -105,3 +105,16 @@ def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_suppor
         out[p] = ker
     out /= out[0].real.sum()
     return out.astype(dtype)
+
+
+def truncate_wkernels(conv, half_supports, support, oversample):
+    """Zero every sample of plane p at |x| >= S_p + 1/2 cells (x measured from the kernel centre
+    (S+1)*OS), in both axes, so plane p's footprint is exactly its central (2 S_p + 1)^2 taps for
+    every fraction index the nearest-rounding w-projection gridder can pick."""
+    conv = np.array(conv, copy=True)
+    csz = conv.shape[-1]
+    x = np.abs(np.arange(csz) / float(oversample) - (support + 1))
+    for p, sp in enumerate(half_supports):
+        keep = x < sp + 0.5
+        conv[p] *= np.outer(keep, keep)
+    return conv

@@ -205,6 +205,12 @@ typedef struct bullseye_options {
 int bullseye_set_options(const bullseye_options *options);
 int bullseye_get_options(bullseye_options *options);
 
+/* Per-plane half supports of the w-projection kernels (plane p uses the central
+ * 2*half_support[p]+1 taps of its (2S+1)^2 footprint; taps outside are taken as zero).  Call
+ * before initLibrary; without it every plane uses conv_support, as in the reference
+ * (gridding_parameters.h:71 has a single support for all planes). */
+int bullseye_set_wplane_support(const uint32_t *half_support, uint32_t nplanes);
+
 /* Zero the device accumulation grids and normalisation sums so a new imaging pass can
  * start without releaseLibrary/initLibrary. */
 int bullseye_reset_grids(void);
