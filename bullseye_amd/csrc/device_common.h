@@ -61,8 +61,8 @@ struct GridArgs {
     const double *tapsum;                      // [OS+1] sum of the 2S+1 taps per fraction
     // outputs
     double *grid;                              // [facet][cube][ny][nx][pstride][2]
-    double *norm;                              // [facet*cube*ncorr]
-    unsigned long long *counters;              // [0] samples, [1] taps
+    double *norm;                              // [nslots][facet*cube*ncorr]
+    unsigned long long *counters;              // [nslots][2]: samples, taps
     int *error_flag;   // bit 0: singular Jones, 1: antenna id, 2: grid slice / spw / time id
     // sizes
     uint32_t nx, ny, S, W, OS, csz;
@@ -71,6 +71,8 @@ struct GridArgs {
     uint32_t imaging_field;
     uint32_t baselines, chan_per_stream, n_chan_blocks, n_segments;
     uint64_t n_streams;
+    uint64_t norm_stride;                      // facet*cube*ncorr (elements of one slot)
+    uint32_t nslots;                           // copies of norm and counters (>= 1)
     double u_scale, v_scale, off_x, off_y;
     int phase, transform;
 };
@@ -110,5 +112,66 @@ template <typename T, bool KAHAN> struct acc1 {
         return KAHAN ? (double)s - (double)c : (double)s;
     }
 };
+
+// ---------- publication of per-lane partial sums ---------------------------------------------
+// Block b adds its normalisation sums and counters into copy (slot) b % nslots; the host folds
+// the slots.  Within a wave, lanes holding the same key are summed before one lane issues the
+// atomic.  Without both, every stream of a launch adds into the same few addresses and the
+// atomics serialise at the cache.  Every lane of the wave must reach these calls.
+__device__ __forceinline__ uint32_t block_slot(const GridArgs &a) { return blockIdx.x % a.nslots; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds v[k] of every lane into norm[slot][key * NC + k]; key < 0 means nothing to add.
+template <int NC>
+__device__ __forceinline__ void publish_norm(const GridArgs &a, int key, const double (&v)[NC]) {
+    const int lane = threadIdx.x & 63;
+    double *base = a.norm + (size_t)block_slot(a) * a.norm_stride;
+    int k = key;
+    for (;;) {
+        const unsigned long long pend = __ballot(k >= 0);
+        if (!pend) break;
+        const int lead = __ffsll((unsigned long long)pend) - 1;
+        const int k0 = __shfl(k, lead, 64);
+        const bool mine = (k == k0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double s = wave_sum(mine ? v[c] : 0.0);
+            if (lane == lead) unsafeAtomicAdd(base + (size_t)k0 * NC + c, s);
+        }
+        if (mine) k = -1;
+    }
+}
+
+__device__ __forceinline__ void publish_counts(const GridArgs &a, unsigned long long samples,
+                                               unsigned long long taps) {
+    const unsigned long long s = wave_sum(samples), t = wave_sum(taps);
+    if ((threadIdx.x & 63) == 0 && s) {
+        unsigned long long *c = a.counters + 2 * (size_t)block_slot(a);
+        atomicAdd(c, s);
+        atomicAdd(c + 1, t);
+    }
+}
+
+// In-stream flush of one lane's sums (a stream's channel moved to another grid slice).
+template <int NC>
+__device__ __forceinline__ void flush_norm_lane(const GridArgs &a, int key, double (&v)[NC]) {
+    double *base = a.norm + (size_t)block_slot(a) * a.norm_stride + (size_t)key * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        unsafeAtomicAdd(base + c, v[c]);
+        v[c] = 0;
+    }
+}
 
 }  // namespace bam

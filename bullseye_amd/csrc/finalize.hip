@@ -162,6 +162,24 @@ hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P
     return hipGetLastError();
 }
 
+// Sums the slot copies of an array into slot 0 and clears the others.
+__global__ void k_fold_slots(double *p, uint64_t stride, uint32_t nslots) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= stride) return;
+    double s = p[i];
+    for (uint32_t j = 1; j < nslots; ++j) {
+        s += p[(uint64_t)j * stride + i];
+        p[(uint64_t)j * stride + i] = 0.0;
+    }
+    p[i] = s;
+}
+
+hipError_t fold_slots(double *p, uint64_t stride, uint32_t nslots, hipStream_t st) {
+    if (!stride || nslots < 2) return hipSuccess;
+    k_fold_slots<<<nblk(stride), 256, 0, st>>>(p, stride, nslots);
+    return hipGetLastError();
+}
+
 hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st) {
     if (!n) return hipSuccess;
     k_fill_f64<<<nblk(n), 256, 0, st>>>(p, n, v);

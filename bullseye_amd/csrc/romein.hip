@@ -196,7 +196,22 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
         }
     }
 
+    // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
+    // early for lanes without a stream, run() then publishes for all of them.
     static __device__ void run(const GridArgs &a, const T *sconv, Rec<T, NC> *srec) {
+        int nkey = -1;
+        double nacc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) nacc[k] = 0;
+        unsigned long long cnt = 0;
+        walk(a, sconv, srec, nkey, nacc, cnt);
+        if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
+        publish_counts(a, cnt, cnt * (unsigned long long)(a.W * a.W));
+    }
+
+    static __device__ __forceinline__ void walk(const GridArgs &a, const T *sconv,
+                                                Rec<T, NC> *srec, int &nkey, double (&nacc)[NC],
+                                                unsigned long long &cnt_samples) {
         const int lane = threadIdx.x & 63;
         const int wave = threadIdx.x >> 6;
         const int W = (int)a.W;
@@ -245,10 +260,6 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
         for (int r = 0; r < R; ++r) py[r] = -1;
         int pcgi = -1, px = -1;
         int ncgi = -1;
-        double nacc[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) nacc[k] = 0;
-        unsigned long long cnt_samples = 0;
 
         for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
             // ---- prepare one sample per lane and publish it -------------------------------
@@ -265,14 +276,7 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
                 prep(a, g, facet, row, c, rec, nw, nval);
                 if (nval && MODE != MODE_PSF) {
                     if (rec.cgi != ncgi) {
-                        if (ncgi >= 0) {
-#pragma unroll
-                            for (int k = 0; k < NC; ++k) {
-                                unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
-                                                nacc[k]);
-                                nacc[k] = 0;
-                            }
-                        }
+                        if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
                         ncgi = rec.cgi;
                     }
 #pragma unroll
@@ -330,15 +334,7 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
             for (int r = 0; r < R; ++r)
                 if (py[r] >= 0) flush(a, facet, pcgi, px, py[r], acc[r]);
         }
-        if (MODE != MODE_PSF && ncgi >= 0) {
-#pragma unroll
-            for (int k = 0; k < NC; ++k)
-                unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k, nacc[k]);
-        }
-        if (cnt_samples) {
-            atomicAdd(a.counters, cnt_samples);
-            atomicAdd(a.counters + 1, cnt_samples * (unsigned long long)(W * W));
-        }
+        nkey = ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1;
     }
 };
 

@@ -285,7 +285,22 @@ template <int MODE, int PATH, int W> struct GridderW {
             }
     }
 
+    // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
+    // early for lanes without a stream, run() then publishes for all of them.
     static __device__ void run(const GridArgs &a, const T *sconv, unsigned char *swave) {
+        int nkey = -1;
+        double nacc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) nacc[k] = 0;
+        unsigned long long cnt = 0;
+        walk(a, sconv, swave, nkey, nacc, cnt);
+        if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
+        publish_counts(a, cnt, cnt * (unsigned long long)(W * W));
+    }
+
+    static __device__ __forceinline__ void walk(const GridArgs &a, const T *sconv,
+                                                unsigned char *swave, int &nkey,
+                                                double (&nacc)[NC], unsigned long long &cnt) {
         const int lane = threadIdx.x & 63;
         const int gw = lane / G;
         const int gl = lane - gw * G;
@@ -324,10 +339,6 @@ template <int MODE, int PATH, int W> struct GridderW {
             }
         int px0 = -1, py0 = -1, pcgi = -1;   // footprint of the accumulated samples
         int ncgi = -1;
-        double nacc[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) nacc[k] = 0;
-        unsigned long long cnt = 0;
 
         for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
             const uint32_t s = step0 + gl;
@@ -344,14 +355,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                         ++cnt;
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
-                                if (ncgi >= 0) {
-#pragma unroll
-                                    for (int k = 0; k < NC; ++k) {
-                                        unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
-                                                        nacc[k]);
-                                        nacc[k] = 0;
-                                    }
-                                }
+                                if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
                                 ncgi = rec.cgi;
                             }
 #pragma unroll
@@ -417,18 +421,7 @@ template <int MODE, int PATH, int W> struct GridderW {
 #pragma unroll
             for (int r = 0; r < W; ++r) flush1(a, facet, pcgi, xo, py0 + mod_w(r - pym), c0, acc[r]);
         }
-        // each lane publishes the normalisation sums and counts of the samples it prepared
-        if constexpr (MODE != MODE_PSF) {
-            if (ncgi >= 0) {
-#pragma unroll
-                for (int k = 0; k < NC; ++k)
-                    unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k, nacc[k]);
-            }
-        }
-        if (cnt) {
-            atomicAdd(a.counters, cnt);
-            atomicAdd(a.counters + 1, cnt * (unsigned long long)(W * W));
-        }
+        nkey = ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1;
     }
 };
 

@@ -46,8 +46,8 @@ struct WArgs {
 
 template <typename T, int NC> struct WRec {
     int x0, y0, xm, ym;     // class-footprint origin and its residues mod M
-    int off, fu, fv, plane; // off = S_M - S_p (first active tap), fraction indexes, plane
-    int cgi, wp, pad0, pad1;  // grid slice; plane width W_p
+    int off, wp, cgi;       // off = S_M - S_p (first active tap), plane width W_p, grid slice
+    uint32_t tb;            // first tap of the sample's W_p x W_p block
     cpx<T> vis[NC];
 };
 
@@ -205,14 +205,13 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         r.xm = r.x0 % (int)w.M;
         r.ym = r.y0 % (int)w.M;
         r.off = sm - sp;
-        r.fu = fu - (int)w.fbase;
-        r.fv = fv - (int)w.fbase;
-        r.plane = plane;
+        const int ru_ = fu - (int)w.fbase, rv_ = fv - (int)w.fbase;
         r.cgi = (int)cgi;
         r.wp = 2 * sp + 1;
+        r.tb = (uint32_t)(w.plane_off[plane] + ((uint64_t)rv_ * w.nf + ru_) * (uint64_t)(r.wp * r.wp));
 #pragma unroll
         for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
-        const double ts = w.plane_norm[((size_t)plane * w.nf + r.fv) * w.nf + r.fu];
+        const double ts = w.plane_norm[((size_t)plane * w.nf + rv_) * w.nf + ru_];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nw[k] = (double)wt[k] * ts;
         return true;
@@ -318,13 +317,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
-                                if (ncgi >= 0)
-#pragma unroll
-                                    for (int k = 0; k < NC; ++k) {
-                                        unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k,
-                                                        nacc[k]);
-                                        nacc[k] = 0;
-                                    }
+                                if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
                                 ncgi = rec.cgi;
                             }
 #pragma unroll
@@ -355,9 +348,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
                     }
                     const int tu = ku - r.off, tv = kv - r.off;
                     if ((unsigned)tu >= (unsigned)r.wp || (unsigned)tv >= (unsigned)r.wp) continue;
-                    const size_t ti = w.plane_off[r.plane] +
-                                      (((size_t)r.fv * w.nf + r.fu) * r.wp + tv) * r.wp + tu;
-                    const cpx<T> tap = tapbase[ti];
+                    const cpx<T> tap = tapbase[r.tb + (uint32_t)(tv * r.wp + tu)];
 #pragma unroll
                     for (int k = 0; k < NC; ++k) {
                         const cpx<T> m = cmul(r.vis[k], tap);
@@ -371,16 +362,10 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p)
             if (own[p] && pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
-        if constexpr (MODE != MODE_PSF) {
-            if (ncgi >= 0)
-#pragma unroll
-                for (int k = 0; k < NC; ++k)
-                    unsafeAtomicAdd(a.norm + ((size_t)facet * a.cube + ncgi) * NC + k, nacc[k]);
-        }
-        if (cnt) {
-            atomicAdd(a.counters, cnt);
-            atomicAdd(a.counters + 1, taps);
-        }
+        // every thread of the block gets here
+        if constexpr (MODE != MODE_PSF)
+            publish_norm<NC>(a, ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1, nacc);
+        publish_counts(a, cnt, taps);
     }
 };
 
@@ -445,7 +430,7 @@ hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, ui
 size_t wproj_record_bytes(int mode, int path) {
     const int nc = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
     const size_t t = path == PATH_FP64 ? 8 : 4;
-    return 48 + 2 * t * nc;
+    return 32 + 2 * t * nc;
 }
 
 hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,

@@ -41,6 +41,7 @@ hipError_t grid_to_abi(const double *g, void *out, uint64_t slices_fc, uint32_t 
 hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P,
                        uint32_t pstride, uint64_t npix, hipStream_t st);
 hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st);
+hipError_t fold_slots(double *p, uint64_t stride, uint32_t nslots, hipStream_t st);
 hipError_t normalize_device(void *out, const double *norm, uint64_t slices, uint64_t npix,
                             hipStream_t st);
 hipError_t weight_uniformly_device(void *out, const void *sf, const uint64_t *cgi, uint32_t F,
@@ -86,6 +87,9 @@ struct State {
     // geometry fixed at initLibrary
     size_t F = 0, cube = 0, P = 0, pstride = 0, nx = 0, ny = 0, S = 0, OS = 0, csz = 0;
     size_t sfC = 0, wplanes = 1;
+    // normalisation sums and counters: nslots copies of norm_stride = F*cube*P (and 2) elements
+    size_t norm_stride = 0;
+    uint32_t nslots = 1;
     bool psf = false;
     int path = PATH_FP64;
     // device buffers
@@ -211,9 +215,15 @@ static int barrier_impl() {
     const bool had_pending = !g.pending.empty();
     g.pending.clear();
     if (had_pending && g.counters.p) {
-        unsigned long long c[2];
-        e = hipMemcpy(c, g.counters.p, sizeof c, hipMemcpyDeviceToHost);
+        std::vector<unsigned long long> cs(2 * (size_t)g.nslots);
+        e = hipMemcpy(cs.data(), g.counters.p, cs.size() * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail("reading counters: %s", hipGetErrorString(e));
+        unsigned long long c[2] = {0, 0};
+        for (size_t i = 0; i < g.nslots; ++i) {
+            c[0] += cs[2 * i];
+            c[1] += cs[2 * i + 1];
+        }
         g.stats.samples_gridded = c[0] - g.last_counters[0];
         g.stats.taps = c[1] - g.last_counters[1];
         g.last_counters[0] = c[0];
@@ -233,6 +243,7 @@ static int publish_results(gridding_parameters *p, bool psf) {
         if (dst)
             HIP_TRY(grid_to_abi(g.grid.as<double>(), dst, g.F * g.cube, (uint32_t)g.P,
                                 (uint32_t)g.pstride, g.nx * g.ny, g.stream));
+        HIP_TRY(fold_slots(g.norm.as<double>(), g.norm_stride, g.nslots, g.stream));
         HIP_TRY(hipMemcpyAsync(g.host_norm.data(), g.norm.p,
                                g.host_norm.size() * sizeof(normalization_base_type),
                                hipMemcpyDeviceToHost, g.stream));
@@ -434,6 +445,8 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.grid = psf ? g.psfgrid.as<double>() : g.grid.as<double>();
     a.norm = g.norm.as<double>();
     a.counters = g.counters.as<unsigned long long>();
+    a.norm_stride = g.norm_stride;
+    a.nslots = g.nslots;
     a.error_flag = g.errflag.as<int>();
     a.nx = (uint32_t)g.nx;
     a.ny = (uint32_t)g.ny;
@@ -590,7 +603,7 @@ int bullseye_reset_grids(void) {
     if (barrier_impl()) return 1;
     HIP_TRY(fill_f64(g.grid.as<double>(), grid_elems(), 0.0, g.stream));
     if (g.psf) HIP_TRY(fill_f64(g.psfgrid.as<double>(), psf_elems(), 0.0, g.stream));
-    HIP_TRY(fill_f64(g.norm.as<double>(), g.host_norm.size(), 0.0, g.stream));
+    HIP_TRY(fill_f64(g.norm.as<double>(), g.norm_stride * g.nslots, 0.0, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     std::fill(g.host_norm.begin(), g.host_norm.end(), 0.0);
     return 0;
@@ -630,15 +643,18 @@ int initLibrary(gridding_parameters *p) {
     if (2 * g.S + 3 >= g.nx || 2 * g.S + 3 >= g.ny) return fail("conv_support too large for the grid");
     if (!p->conv) return fail("conv is NULL");
     HIP_TRY(g.grid.ensure(grid_elems() * sizeof(double)));
-    HIP_TRY(g.norm.ensure(g.F * g.cube * g.P * sizeof(double)));
-    HIP_TRY(g.counters.ensure(2 * sizeof(unsigned long long)));
+    g.norm_stride = g.F * g.cube * g.P;
+    g.nslots = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, ((size_t)1 << 22) / g.norm_stride));
+    if (g.norm_stride > 0x7fffffffull) return fail("too many grid slices");
+    HIP_TRY(g.norm.ensure(g.norm_stride * g.nslots * sizeof(double)));
+    HIP_TRY(g.counters.ensure(2 * (size_t)g.nslots * sizeof(unsigned long long)));
     HIP_TRY(g.errflag.ensure(sizeof(int)));
-    HIP_TRY(hipMemset(g.counters.p, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(g.counters.p, 0, 2 * (size_t)g.nslots * sizeof(unsigned long long)));
     HIP_TRY(hipMemset(g.errflag.p, 0, sizeof(int)));
     g.last_counters[0] = g.last_counters[1] = 0;
     g.host_norm.assign(g.F * g.cube * g.P, 0.0);
     p->normalization_terms = g.host_norm.data();
-    HIP_TRY(fill_f64(g.norm.as<double>(), g.host_norm.size(), 0.0, g.stream));
+    HIP_TRY(fill_f64(g.norm.as<double>(), g.norm_stride * g.nslots, 0.0, g.stream));
     // seed the accumulation grid from the caller's buffer (gpu_algorithm/wrapper.cu:165-168)
     const void *src = nullptr;
     if (p->output_buffer) {
@@ -719,6 +735,7 @@ int initLibrary(gridding_parameters *p) {
             const size_t wq = 2 * g.wplane_half_host[q] + 1;
             off[q + 1] = off[q] + nf * nf * wq * wq;
         }
+        if (off[P] > 0xffffffffull) return fail("w-projection tap table exceeds 2^32 taps");
         const size_t nconv = P * csz * csz;
         std::vector<convolution_base_type> ch(2 * nconv);
         if (dev_mode())
@@ -828,6 +845,7 @@ int normalize(gridding_parameters *p) {
     if (barrier_impl()) return 1;
     const size_t slices = g.F * g.cube * g.P, npix = p->nx * p->ny;
     if (dev_mode()) {
+        HIP_TRY(fold_slots(g.norm.as<double>(), g.norm_stride, g.nslots, g.stream));
         HIP_TRY(normalize_device(p->output_buffer, g.norm.as<double>(), slices, npix, g.stream));
         HIP_TRY(hipStreamSynchronize(g.stream));
         return 0;

@@ -71,20 +71,24 @@ def grid_taper(conv, support, oversample, n):
     return t / conv.sum()
 
 
-def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_support=None,
-                  dtype=np.complex128):
+def linear_plane_supports(wplanes, s_min, s_max):
+    """Half supports growing linearly with the plane index (|w|), s_min at w = 0."""
+    p = np.arange(wplanes)
+    return [int(x) for x in np.rint(s_min + (s_max - s_min) * p / max(wplanes - 1, 1))]
+
+
+def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_support=3,
+                  half_supports=None, dtype=np.complex128):
     """2D w-projection kernels in the reference's layout complex[wplanes][csz][csz].
 
-    Plane p holds the uv-plane kernel for w_p = p * wmax / (wplanes - 1) (the reference's plane
-    choice is lrint(|w| / wmax * (wplanes - 1)), convolution_policies.h:264).  Each kernel is the
-    Fourier transform of the image-plane product PSWF_taper(l) PSWF_taper(m) e^{-2 pi i w_p (n-1)}
-    evaluated on an oversampled image grid, cut to the (2S+3)-cell padded support, and
-    normalised so the zero-w plane sums to one (as the AA kernel).
+    Plane p is the kernel for w_p = p * wmax / (wplanes - 1) (the gridder picks
+    lrint(|w| / wmax * (wplanes - 1)), convolution_policies.h:264): the inverse Fourier transform
+    of T(l) T(m) exp(-2 pi i w_p (n - 1)) sampled on the image grid that the oversampled kernel's
+    DFT spans (OS times the field of view), where T is the image-plane response of the PSWF AA
+    kernel of half support aa_support.  Planes are normalised so plane 0 sums to one, then cut
+    to their half supports (kernels.truncate_wkernels) when half_supports is given.
     """
-    aa_support = support if aa_support is None else aa_support
     csz = aa_kernel_size(support, oversample)
-    # image-plane sampling: the kernel spans csz samples at 1/OS cell spacing; its transform
-    # is sampled on an image grid of csz points covering OS times the field of view
     fov = nx * cell_arcsec * np.pi / 648000.0
     k = np.arange(csz) - csz // 2
     lm = k / float(csz) * fov * oversample
@@ -95,15 +99,15 @@ def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_suppor
     taper1 = np.real(np.fft.fftshift(np.fft.fft(np.fft.ifftshift(aa_pad))))
     taper2 = np.outer(taper1, taper1)
     lg, mg = np.meshgrid(lm, lm, indexing="ij")
-    r2 = np.minimum(lg ** 2 + mg ** 2, 0.99)
-    nm1 = np.sqrt(1.0 - r2) - 1.0
+    nm1 = np.sqrt(np.maximum(1.0 - lg ** 2 - mg ** 2, 0.0)) - 1.0
     out = np.empty((wplanes, csz, csz), dtype=np.complex128)
     step = wmax / float(max(wplanes - 1, 1))
     for p in range(wplanes):
         img = taper2 * np.exp(-2j * np.pi * (p * step) * nm1)
-        ker = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(img)))
-        out[p] = ker
+        out[p] = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(img)))
     out /= out[0].real.sum()
+    if half_supports is not None:
+        out = truncate_wkernels(out, half_supports, support, oversample)
     return out.astype(dtype)
 
 

@@ -97,7 +97,9 @@ def lm_to_radec(l, m, ra0, dec0):
 def simulate(n_ant=64, array="disc", array_size_m=3000.0, n_time=240, hours=8.0,
              dec_deg=-30.0, ra_deg=45.0, latitude_deg=-30.7, f_lo=1.40e9, f_hi=1.42e9,
              n_chan=16, n_corr=1, n_src=20, src_radius_deg=0.5, flux=(0.1, 1.0), noise=0.05,
-             flag_fraction=0.05, jones_sigma=None, seed=0, predict_chunk=32768):
+             flag_fraction=0.05, jones_sigma=None, seed=0, predict_chunk=32768, device=None):
+    """device: optional torch device on which the point-source prediction runs (float64); the
+    result equals the NumPy prediction to rounding."""
     rng = np.random.default_rng(seed)
     if array == "disc":
         enu = antennas_disc(n_ant, array_size_m, rng)
@@ -128,11 +130,24 @@ def simulate(n_ant=64, array="disc", array_size_m=3000.0, n_time=240, hours=8.0,
     # predict in float64: phase per (row, source) in metres, then per channel
     model = np.empty((rows, n_chan), dtype=np.complex128)
     k = 2 * np.pi * freqs / C_LIGHT
-    for r0 in range(0, rows, predict_chunk):
-        r1 = min(rows, r0 + predict_chunk)
-        d = uvw[r0:r1, 0:1] * ls + uvw[r0:r1, 1:2] * ms + uvw[r0:r1, 2:3] * ns
-        ph = np.exp(1j * d[:, None, :] * k[None, :, None])
-        model[r0:r1] = ph @ fl
+    if device is not None:
+        import torch
+        kt = torch.as_tensor(k, device=device)
+        lt, mt, nt = (torch.as_tensor(x, device=device) for x in (ls, ms, ns))
+        ft = torch.as_tensor(fl, device=device, dtype=torch.complex128)
+        step = max(predict_chunk, 1 << 18)
+        for r0 in range(0, rows, step):
+            r1 = min(rows, r0 + step)
+            u = torch.as_tensor(uvw[r0:r1], device=device)
+            d = u[:, 0:1] * lt + u[:, 1:2] * mt + u[:, 2:3] * nt
+            ph = torch.exp(1j * (d[:, None, :] * kt[None, :, None]))
+            model[r0:r1] = (ph @ ft).cpu().numpy()
+    else:
+        for r0 in range(0, rows, predict_chunk):
+            r1 = min(rows, r0 + predict_chunk)
+            d = uvw[r0:r1, 0:1] * ls + uvw[r0:r1, 1:2] * ms + uvw[r0:r1, 2:3] * ns
+            ph = np.exp(1j * d[:, None, :] * k[None, :, None])
+            model[r0:r1] = ph @ fl
     vis = np.zeros((rows, n_chan, n_corr), dtype=np.complex128)
     if n_corr == 1:
         vis[..., 0] = model
