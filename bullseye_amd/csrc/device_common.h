@@ -59,6 +59,7 @@ struct GridArgs {
     const FacetGeom *geom;                     // [facet]
     const void *conv;                          // AA taps (acc_t) [csz]
     const double *tapsum;                      // [OS+1] sum of the 2S+1 taps per fraction
+    const int *time_inner;                     // [1] walk order chosen on the device
     // outputs
     double *grid;                              // [facet][cube][ny][nx][pstride][2]
     double *norm;                              // [nslots][facet*cube*ncorr]
@@ -112,6 +113,22 @@ template <typename T, bool KAHAN> struct acc1 {
         return KAHAN ? (double)s - (double)c : (double)s;
     }
 };
+
+// Sample s of a stream of nrows rows x ncb channels -> (row offset tr, channel offset j).
+// Channels inner: rows in order, channels serpentine; time inner: channels in order, rows
+// serpentine.  Either way consecutive samples are neighbours in (time, frequency).
+__device__ __forceinline__ void stream_sample(uint32_t s, uint32_t nrows, uint32_t ncb,
+                                              bool time_inner, uint32_t &tr, uint32_t &j) {
+    if (time_inner) {
+        j = s / nrows;
+        tr = s - j * nrows;
+        if (j & 1) tr = nrows - 1 - tr;
+    } else {
+        tr = s / ncb;
+        j = s - tr * ncb;
+        if (tr & 1) j = ncb - 1 - j;
+    }
+}
 
 // ---------- publication of per-lane partial sums ---------------------------------------------
 // Block b adds its normalisation sums and counters into copy (slot) b % nslots; the host folds

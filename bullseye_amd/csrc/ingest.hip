@@ -82,4 +82,78 @@ hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, ui
     return hipcub::DeviceScan::ExclusiveSum(ws, b2, counts, bl_start, (int)n_bl + 1, st);
 }
 
+// Walk order of the Romein streams: the mean footprint movement, |du| + |dv| in grid cells,
+// per time step (consecutive rows of a baseline at the middle channel) and per channel step
+// (the middle channel pair), over up to 4096 baselines x 32 rows.  Time inner when a time step
+// moves the footprint less: that order flushes fewer grid points.  One workgroup; the result
+// stays on the device for the gridding kernels.  override_order: 1 channels inner, 2 time
+// inner, anything else measured.
+constexpr int WALK_THREADS = 1024;
+
+__global__ __launch_bounds__(WALK_THREADS) void k_walk_order(GridArgs a, int override_order,
+                                                             int *time_inner) {
+    __shared__ double s_d[2][WALK_THREADS];
+    __shared__ uint32_t s_n[2][WALK_THREADS];
+    double dt = 0, dc = 0;
+    uint32_t nt = 0, nc = 0;
+    if (override_order != 1 && override_order != 2 && a.chan > 1) {
+        const uint32_t B = a.baselines;
+        const uint32_t nbs = B < 4096u ? B : 4096u;
+        const uint32_t cm = (a.chan - 1) / 2, cn = cm + 1;
+        for (uint32_t i = threadIdx.x; i < nbs; i += WALK_THREADS) {
+            const uint32_t b = (uint32_t)((uint64_t)i * B / nbs);
+            const uint32_t r0 = a.bl_start[b];
+            const uint32_t r1 = min(a.bl_start[b + 1], r0 + 33);
+            double pu = 0, pv = 0;
+            bool have = false;
+            for (uint32_t r = r0; r < r1; ++r) {
+                const uint32_t row = a.perm[r];
+                const uint32_t s = a.spw[row];
+                if (s >= a.spw_count) continue;
+                const double u = (double)a.uvw[3 * (size_t)row] * a.u_scale;
+                const double v = (double)a.uvw[3 * (size_t)row + 1] * a.v_scale;
+                const double rl = 1.0 / (double)a.wavelengths[s * a.chan + cm];
+                const double rl2 = 1.0 / (double)a.wavelengths[s * a.chan + cn];
+                if (have) {
+                    dt += fabs(u * rl - pu) + fabs(v * rl - pv);
+                    ++nt;
+                }
+                pu = u * rl;
+                pv = v * rl;
+                have = true;
+                dc += (fabs(u) + fabs(v)) * fabs(rl2 - rl);
+                ++nc;
+            }
+        }
+    }
+    s_d[0][threadIdx.x] = dt;
+    s_d[1][threadIdx.x] = dc;
+    s_n[0][threadIdx.x] = nt;
+    s_n[1][threadIdx.x] = nc;
+    __syncthreads();
+    for (int h = WALK_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s_d[0][threadIdx.x] += s_d[0][threadIdx.x + h];
+            s_d[1][threadIdx.x] += s_d[1][threadIdx.x + h];
+            s_n[0][threadIdx.x] += s_n[0][threadIdx.x + h];
+            s_n[1][threadIdx.x] += s_n[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int ti = 0;
+        if (override_order == 2)
+            ti = 1;
+        else if (override_order != 1 && s_n[0][0] && s_n[1][0])
+            ti = s_d[0][0] / s_n[0][0] < s_d[1][0] / s_n[1][0] ? 1 : 0;
+        *time_inner = ti;
+    }
+}
+
+hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
+                             hipStream_t st) {
+    k_walk_order<<<1, WALK_THREADS, 0, st>>>(a, override_order, time_inner);
+    return hipGetLastError();
+}
+
 }  // namespace bam

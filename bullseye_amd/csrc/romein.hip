@@ -240,7 +240,9 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
         const uint32_t c_lo = cb * a.chan_per_stream;
         const uint32_t c_hi = min(a.chan, c_lo + a.chan_per_stream);
         const uint32_t ncb = c_hi - c_lo;
-        const uint32_t nsamp = (r_hi - r_lo) * ncb;
+        const uint32_t nrows = r_hi - r_lo;
+        const uint32_t nsamp = nrows * ncb;
+        const bool tin = *a.time_inner != 0;
         const FacetGeom g = a.geom[facet];
 
         const int ua = gl % W;   // u residue of this lane
@@ -267,9 +269,9 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
             Rec<T, NC> rec;
             rec.cgi = -1;
             if (s < nsamp) {
-                const uint32_t tr = s / ncb;
-                const uint32_t j = s - tr * ncb;
-                const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
+                uint32_t tr, j;
+                stream_sample(s, nrows, ncb, tin, tr, j);
+                const uint32_t c = c_lo + j;
                 const uint32_t row = a.perm[b0 + r_lo + tr];
                 double nw[NC];
                 int nval;

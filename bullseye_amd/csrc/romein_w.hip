@@ -322,7 +322,9 @@ template <int MODE, int PATH, int W> struct GridderW {
         const uint32_t c_lo = cb * a.chan_per_stream;
         const uint32_t c_hi = min(a.chan, c_lo + a.chan_per_stream);
         const uint32_t ncb = c_hi - c_lo;
-        const uint32_t nsamp = (r_hi - r_lo) * ncb;
+        const uint32_t nrows = r_hi - r_lo;
+        const uint32_t nsamp = nrows * ncb;
+        const bool tin = *a.time_inner != 0;
         const FacetGeom *gp = a.geom + facet;
 
         const int ua = gl % W;          // u residue of this lane
@@ -346,9 +348,9 @@ template <int MODE, int PATH, int W> struct GridderW {
                 R_t rec;
                 rec.cgi = -1;
                 if (s < nsamp) {
-                    const uint32_t tr = s / ncb;
-                    const uint32_t j = s - tr * ncb;
-                    const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
+                    uint32_t tr, j;
+                    stream_sample(s, nrows, ncb, tin, tr, j);
+                    const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
                     if (prep(a, gp, sconv, facet, row, c, rec, nw)) {

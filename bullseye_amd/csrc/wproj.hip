@@ -84,14 +84,14 @@ __global__ void k_stream_class(GridArgs a, const uint32_t *plane_half, uint32_t 
     cls[s] = (uint8_t)(r_lo < r_hi ? best : 0);
 }
 
-template <int MODE, int PATH, int PAIRS> struct WGridder {
+// Preparation of one sample for a class of width M: the record and the normalisation terms.
+template <int MODE, int PATH> struct WPrep {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
     typedef typename PathTraits<PATH>::acc_t T;
-    static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
     typedef WRec<T, NC> R_t;
 
-    static __device__ __forceinline__ bool prep(const WArgs &w, uint32_t facet, uint32_t row,
-                                                uint32_t c, R_t &r, double nw[NC]) {
+    static __device__ __forceinline__ bool prep(const WArgs &w, int M, uint32_t facet,
+                                                uint32_t row, uint32_t c, R_t &r, double nw[NC]) {
         const GridArgs &a = w.g;
         const uint32_t s = a.spw[row];
         if (s >= a.spw_count) {
@@ -198,12 +198,12 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
             return false;
         const int plane = (int)pl;
         const int sp = (int)w.plane_half[plane];
-        const int sm = ((int)w.M - 1) / 2;
+        const int sm = (M - 1) / 2;
         // class footprint [x0, x0 + M) is centred where the reference's taps are
         r.x0 = dx + (int)a.S - sm;
         r.y0 = dy + (int)a.S - sm;
-        r.xm = r.x0 % (int)w.M;
-        r.ym = r.y0 % (int)w.M;
+        r.xm = r.x0 % M;
+        r.ym = r.y0 % M;
         r.off = sm - sp;
         const int ru_ = fu - (int)w.fbase, rv_ = fv - (int)w.fbase;
         r.cgi = (int)cgi;
@@ -216,6 +216,13 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         for (int k = 0; k < NC; ++k) nw[k] = (double)wt[k] * ts;
         return true;
     }
+};
+
+template <int MODE, int PATH, int PAIRS> struct WGridder {
+    static constexpr int NC = ModeTraits<MODE>::NCORR;
+    typedef typename PathTraits<PATH>::acc_t T;
+    static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
+    typedef WRec<T, NC> R_t;
 
     static __device__ __forceinline__ void flush(const GridArgs &a, uint32_t facet, int cgi,
                                                  int x, int y, acc1<T, KAHAN> (&acc)[NC][2]) {
@@ -252,7 +259,8 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         if (threadIdx.x == 0) s_steps = 0;
         __syncthreads();
         // stream geometry
-        uint32_t facet = 0, b0 = 0, r_lo = 0, c_lo = 0, ncb = 1, nsamp = 0;
+        uint32_t facet = 0, b0 = 0, r_lo = 0, c_lo = 0, ncb = 1, nsamp = 0, nrows = 0;
+        const bool tin = *a.time_inner != 0;
         if (live) {
             uint64_t t = w.stream_list[li];
             facet = (uint32_t)(t % a.facets);
@@ -267,7 +275,8 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
             const uint32_t r_hi = min(nb, r_lo + SEG);
             c_lo = cb * a.chan_per_stream;
             ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
-            nsamp = r_hi > r_lo ? (r_hi - r_lo) * ncb : 0;
+            nrows = r_hi > r_lo ? r_hi - r_lo : 0;
+            nsamp = nrows * ncb;
             if (sl == 0) atomicMax(&s_steps, (nsamp + REC - 1) / REC);
         }
         __syncthreads();
@@ -307,12 +316,12 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
                 rec.cgi = -1;
                 const uint32_t s = step0 + sl;
                 if (s < nsamp) {
-                    const uint32_t tr = s / ncb;
-                    const uint32_t j = s - tr * ncb;
-                    const uint32_t c = c_lo + ((tr & 1) ? (ncb - 1 - j) : j);
+                    uint32_t tr, j;
+                    stream_sample(s, nrows, ncb, tin, tr, j);
+                    const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (prep(w, facet, row, c, rec, nw)) {
+                    if (WPrep<MODE, PATH>::prep(w, M, facet, row, c, rec, nw)) {
                         ++cnt;
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -369,6 +378,333 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
     }
 };
 
+// ---------- column ownership for narrow classes ----------------------------------------------
+// For M <= WCOL_MAX a group of M * NSPLIT lanes walks one stream inside a wavefront, the AA
+// gridder's layout: lane (h, a) owns u-residue a and correlations [h*CPL, (h+1)*CPL) of all M
+// v-residues.  Per sample a lane issues its M tap loads together and does M complex MACs, so
+// the per-sample bookkeeping (record read, ownership move) is shared by M taps and the loads'
+// latency overlaps.  FP32 compensated sums take one Kahan step per run of QUAD samples on one
+// footprint (products chained by FMA onto -c).
+constexpr int WCOL_MAX = 15;
+constexpr int WCOL_BLOCK = 256;
+constexpr int QUAD = 4;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int MODE, int PATH, int M> struct WColGridder {
+    static constexpr int NC = ModeTraits<MODE>::NCORR;
+    static constexpr int CPL = NC == 4 ? (PATH == PATH_FP64 ? 2 : 1) : NC;
+    static constexpr int NSPLIT = NC / CPL;
+    static constexpr int G = M * NSPLIT;     // lanes per group
+    static constexpr int GPW = 64 / G;       // groups per wavefront
+    typedef typename PathTraits<PATH>::acc_t T;
+    static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
+    typedef WRec<T, NC> R_t;
+    // record arrays of consecutive groups start 8 LDS banks apart
+    static constexpr int GRAW = (G * (int)sizeof(R_t) + 15) & ~15;
+    static constexpr int GSTRIDE = GRAW + ((8 - (GRAW / 4) % 32 + 32) % 32) * 4;
+    static constexpr int WAVE_BYTES = GPW * GSTRIDE;
+    typedef acc1<T, KAHAN> Acc[M][CPL][2];
+
+    static __device__ __forceinline__ int mod_m(int v) {  // v in (-M, 2M)
+        v = v < 0 ? v + M : v;
+        return v >= M ? v - M : v;
+    }
+
+    static __device__ __forceinline__ void flush1(const GridArgs &a, uint32_t facet, int cgi,
+                                                  int x, int y, int c0,
+                                                  acc1<T, KAHAN> (&acc)[CPL][2]) {
+        bool nz = false;
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) nz |= (acc[k][0].s != T(0)) | (acc[k][1].s != T(0));
+        if (nz) {
+            double *cell = a.grid + (((((size_t)facet * a.cube + cgi) * a.ny + y) * a.nx + x) *
+                                         (size_t)a.pstride + c0) * 2;
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                unsafeAtomicAdd(cell + 2 * k, acc[k][0].value());
+                unsafeAtomicAdd(cell + 2 * k + 1, acc[k][1].value());
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            acc[k][0].zero();
+            acc[k][1].zero();
+        }
+    }
+
+    // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.
+    static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int ua,
+                                                   int c0, int x0, int y0, int cgi, int &px0,
+                                                   int &py0, int &pcgi, Acc &acc) {
+        if (x0 == px0 && y0 == py0 && cgi == pcgi) return;
+        if (pcgi >= 0) {
+            const int xo = px0 + mod_m(ua - px0 % M);
+            const int xn = x0 + mod_m(ua - x0 % M);
+            const bool all = (cgi != pcgi) || (xn != xo);
+            const int pym = py0 % M, ym = y0 % M;
+#pragma unroll
+            for (int r = 0; r < M; ++r) {
+                const int yo = py0 + mod_m(r - pym);
+                const int yn = y0 + mod_m(r - ym);
+                if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+            }
+        }
+        px0 = x0;
+        py0 = y0;
+        pcgi = cgi;
+    }
+
+    // The lane's column of the sample's taps in v-residue order (zero outside the plane's
+    // W_p x W_p window); false when the lane's column is outside the window.
+    static __device__ __forceinline__ bool column(const R_t &rec, const cpx<T> *taps, int ua,
+                                                  cpx<T> (&tp)[M]) {
+        const int tu = mod_m(ua - rec.xm) - rec.off;
+        if ((unsigned)tu >= (unsigned)rec.wp) return false;
+        const uint32_t base = rec.tb + (uint32_t)tu;
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            const int tv = mod_m(r - rec.ym) - rec.off;
+            const bool ok = (unsigned)tv < (unsigned)rec.wp;
+            const cpx<T> t = taps[ok ? base + (uint32_t)(tv * rec.wp) : rec.tb];
+            tp[r] = ok ? t : cpx<T>{T(0), T(0)};
+        }
+        return true;
+    }
+
+    static __device__ __forceinline__ void add_one(const R_t &rec, const cpx<T> *taps, int ua,
+                                                   int c0, Acc &acc) {
+        cpx<T> tp[M];
+        if (!column(rec, taps, ua, tp)) return;
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                const cpx<T> v = rec.vis[c0 + k];
+                const T re = fma(v.re, tp[r].re, -v.im * tp[r].im);
+                const T im = fma(v.re, tp[r].im, v.im * tp[r].re);
+                if constexpr (KAHAN) {
+                    acc[r][k][0].add_pre(re - acc[r][k][0].c);
+                    acc[r][k][1].add_pre(im - acc[r][k][1].c);
+                } else {
+                    acc[r][k][0].s += re;
+                    acc[r][k][1].s += im;
+                }
+            }
+    }
+
+    static __device__ __forceinline__ void add_quad(const R_t *recs, const cpx<T> *taps, int ua,
+                                                    int c0, Acc &acc) {
+        T y[M][CPL][2];
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                y[r][k][0] = -acc[r][k][0].c;
+                y[r][k][1] = -acc[r][k][1].c;
+            }
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < QUAD; ++q) {
+            const R_t &rec = recs[q];
+            cpx<T> tp[M];
+            if (!column(rec, taps, ua, tp)) continue;
+            any = true;
+#pragma unroll
+            for (int r = 0; r < M; ++r)
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) {
+                    const cpx<T> v = rec.vis[c0 + k];
+                    y[r][k][0] = fma(v.re, tp[r].re, fma(-v.im, tp[r].im, y[r][k][0]));
+                    y[r][k][1] = fma(v.re, tp[r].im, fma(v.im, tp[r].re, y[r][k][1]));
+                }
+        }
+        if (!any) return;
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                acc[r][k][0].add_pre(y[r][k][0]);
+                acc[r][k][1].add_pre(y[r][k][1]);
+            }
+    }
+
+    static __device__ void run(const WArgs &w, unsigned char *swave) {
+        const GridArgs &a = w.g;
+        int nkey = -1;
+        double nacc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) nacc[k] = 0;
+        unsigned long long cnt = 0, taps = 0;
+        walk(w, swave, nkey, nacc, cnt, taps);
+        if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
+        publish_counts(a, cnt, taps);
+    }
+
+    static __device__ __forceinline__ void walk(const WArgs &w, unsigned char *swave, int &nkey,
+                                                double (&nacc)[NC], unsigned long long &cnt,
+                                                unsigned long long &ntaps) {
+        const GridArgs &a = w.g;
+        const int lane = threadIdx.x & 63;
+        const int gw = lane / G;
+        const int gl = lane - gw * G;
+        if (gw >= GPW) return;
+        const uint64_t li =
+            ((uint64_t)blockIdx.x * (WCOL_BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
+        if (li >= w.n_list) return;
+        uint64_t t = w.stream_list[li];
+        const uint32_t facet = (uint32_t)(t % a.facets);
+        t /= a.facets;
+        const uint32_t seg = (uint32_t)(t % a.n_segments);
+        t /= a.n_segments;
+        const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
+        const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
+        const uint32_t b0 = a.bl_start[bl];
+        const uint32_t nb = a.bl_start[bl + 1] - b0;
+        const uint32_t r_lo = seg * SEG;
+        const uint32_t r_hi = min(nb, r_lo + SEG);
+        const uint32_t c_lo = cb * a.chan_per_stream;
+        const uint32_t ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
+        const uint32_t nrows = r_hi > r_lo ? r_hi - r_lo : 0;
+        const uint32_t nsamp = nrows * ncb;
+        const bool tin = *a.time_inner != 0;
+
+        const int ua = gl % M;          // u residue of this lane
+        const int c0 = (gl / M) * CPL;  // first correlation of this lane
+        R_t *grp = reinterpret_cast<R_t *>(swave + gw * GSTRIDE);
+        const cpx<T> *taps = reinterpret_cast<const cpx<T> *>(w.taps);
+
+        Acc acc;
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                acc[r][k][0].zero();
+                acc[r][k][1].zero();
+            }
+        int px0 = -1, py0 = -1, pcgi = -1;
+        int ncgi = -1;
+
+        for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
+            const uint32_t s = step0 + gl;
+            {
+                R_t rec;
+                rec.cgi = -1;
+                if (s < nsamp) {
+                    uint32_t tr, j;
+                    stream_sample(s, nrows, ncb, tin, tr, j);
+                    const uint32_t c = c_lo + j;
+                    const uint32_t row = a.perm[b0 + r_lo + tr];
+                    double nw[NC];
+                    if (WPrep<MODE, PATH>::prep(w, M, facet, row, c, rec, nw)) {
+                        ++cnt;
+                        ntaps += (unsigned long long)(rec.wp * rec.wp);
+                        if constexpr (MODE != MODE_PSF) {
+                            if (rec.cgi != ncgi) {
+                                if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
+                                ncgi = rec.cgi;
+                            }
+#pragma unroll
+                            for (int k = 0; k < NC; ++k) nacc[k] += nw[k];
+                        }
+                    } else {
+                        rec.cgi = -1;
+                    }
+                }
+                if (rec.cgi < 0) {
+                    // contributes nothing: no column of it is inside its (empty) window
+                    rec.x0 = rec.y0 = rec.xm = rec.ym = 0;
+                    rec.off = 0;
+                    rec.wp = 0;
+                    rec.tb = 0;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) rec.vis[k] = {T(0), T(0)};
+                }
+                grp[gl] = rec;
+            }
+            wave_sync();
+            const int kmax = (int)min((uint32_t)G, nsamp - step0);
+            int kk = 0;
+            if constexpr (KAHAN) {
+                while (kk + QUAD <= kmax) {
+                    int cg = -1, x0 = 0, y0 = 0;
+                    bool same = true;
+#pragma unroll
+                    for (int q = 0; q < QUAD; ++q) {
+                        const int c1 = grp[kk + q].cgi;
+                        if (c1 < 0) continue;
+                        const int x1 = grp[kk + q].x0, y1 = grp[kk + q].y0;
+                        if (cg < 0) {
+                            cg = c1;
+                            x0 = x1;
+                            y0 = y1;
+                        } else {
+                            same = same && c1 == cg && x1 == x0 && y1 == y0;
+                        }
+                    }
+                    if (!same) break;
+                    if (cg >= 0) {
+                        move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
+                        add_quad(grp + kk, taps, ua, c0, acc);
+                    }
+                    kk += QUAD;
+                }
+            }
+            for (; kk < kmax; ++kk) {
+                const int cg = grp[kk].cgi;
+                if (cg < 0) continue;
+                move_to(a, facet, ua, c0, grp[kk].x0, grp[kk].y0, cg, px0, py0, pcgi, acc);
+                add_one(grp[kk], taps, ua, c0, acc);
+            }
+            wave_sync();
+        }
+        if (pcgi >= 0) {
+            const int xo = px0 + mod_m(ua - px0 % M);
+            const int pym = py0 % M;
+#pragma unroll
+            for (int r = 0; r < M; ++r) flush1(a, facet, pcgi, xo, py0 + mod_m(r - pym), c0, acc[r]);
+        }
+        nkey = ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1;
+    }
+};
+
+template <int MODE, int PATH, int M>
+__global__ __launch_bounds__(WCOL_BLOCK) void wcol_kernel(WArgs w) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    typedef WColGridder<MODE, PATH, M> Gd;
+    Gd::run(w, smem + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
+}
+
+template <int MODE, int PATH, int M>
+static hipError_t launch_col(const WArgs &w, hipStream_t st) {
+    typedef WColGridder<MODE, PATH, M> Gd;
+    const uint64_t waves = (w.n_list + Gd::GPW - 1) / Gd::GPW;
+    const uint64_t blocks = (waves + WCOL_BLOCK / 64 - 1) / (WCOL_BLOCK / 64);
+    if (!blocks) return hipSuccess;
+    const size_t smem = (size_t)(WCOL_BLOCK / 64) * Gd::WAVE_BYTES;
+    wcol_kernel<MODE, PATH, M><<<dim3((uint32_t)blocks), dim3(WCOL_BLOCK), smem, st>>>(w);
+    return hipGetLastError();
+}
+
+template <int MODE, int PATH>
+static hipError_t launch_col_m(const WArgs &w, hipStream_t st) {
+    switch (w.M) {
+        case 1: return launch_col<MODE, PATH, 1>(w, st);
+        case 3: return launch_col<MODE, PATH, 3>(w, st);
+        case 5: return launch_col<MODE, PATH, 5>(w, st);
+        case 7: return launch_col<MODE, PATH, 7>(w, st);
+        case 9: return launch_col<MODE, PATH, 9>(w, st);
+        case 11: return launch_col<MODE, PATH, 11>(w, st);
+        case 13: return launch_col<MODE, PATH, 13>(w, st);
+        case 15: return launch_col<MODE, PATH, 15>(w, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int MODE, int PATH, int PAIRS>
 __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -397,6 +733,14 @@ static hipError_t launch_p(const WArgs &w, int pairs, uint32_t block, hipStream_
 
 template <int MODE>
 static hipError_t launch_m(const WArgs &w, int path, int pairs, uint32_t block, hipStream_t st) {
+    if (w.M <= (uint32_t)WCOL_MAX) {
+        switch (path) {
+            case PATH_FP64: return launch_col_m<MODE, PATH_FP64>(w, st);
+            case PATH_F32K: return launch_col_m<MODE, PATH_F32K>(w, st);
+            case PATH_F32: return launch_col_m<MODE, PATH_F32>(w, st);
+        }
+        return hipErrorInvalidValue;
+    }
     switch (path) {
         case PATH_FP64: return launch_p<MODE, PATH_FP64>(w, pairs, block, st);
         case PATH_F32K: return launch_p<MODE, PATH_F32K>(w, pairs, block, st);

@@ -31,6 +31,8 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
                         uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
                         hipStream_t st);
 int romein_max_width();
+hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
+                             hipStream_t st);
 size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
 hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
                        uint32_t n_bl, uint32_t *keys, uint32_t *vals, uint32_t *keys_out,
@@ -97,7 +99,7 @@ struct State {
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
-    DevBuf out_tmp, fft_scratch, fft_real, jinv;
+    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
     DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
@@ -228,6 +230,12 @@ static int barrier_impl() {
         g.stats.taps = c[1] - g.last_counters[1];
         g.last_counters[0] = c[0];
         g.last_counters[1] = c[1];
+        if (g.walk.p) {
+            int ti = 0;
+            e = hipMemcpy(&ti, g.walk.p, sizeof ti, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return fail("reading the walk order: %s", hipGetErrorString(e));
+            g.stats.time_inner = (uint32_t)ti;
+        }
     }
     return check_device_errors();
 }
@@ -481,6 +489,9 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.off_y = (double)(p->ny / 2 - p->conv_support);
     a.phase = phase ? 1 : 0;
     a.transform = facet ? 1 : 0;
+    HIP_TRY(g.walk.ensure(sizeof(int)));
+    a.time_inner = g.walk.as<int>();
+    HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, g.walk.as<int>(), g.stream));
 
     if (wproj) {
         // w-projection: streams are SEG-row segments bucketed by their support class
@@ -564,6 +575,7 @@ int bullseye_set_options(const bullseye_options *o) {
     if (o->struct_size != sizeof(bullseye_options))
         return fail("bullseye_options.struct_size %u != %zu", o->struct_size, sizeof(bullseye_options));
     if (o->compute_path > BULLSEYE_PATH_FP32) return fail("bad compute_path %u", o->compute_path);
+    if (o->walk_order > 2) return fail("bad walk_order %u", o->walk_order);
     g.opts = *o;
     if (g.initialized)
         g.path = o->compute_path ? (int)o->compute_path
@@ -726,7 +738,10 @@ int initLibrary(gridding_parameters *p) {
                 g.wplane_half_host[q] = g.wplane_half_req[q];
             }
         }
-        if (2 * S + 1 > 255) return fail("w-projection support too large");
+        for (size_t q = 0; q < P; ++q)
+            if (g.wplane_half_host[q] > 31)
+                return fail("w-projection plane %zu: half support %u exceeds 31", q,
+                            g.wplane_half_host[q]);
         g.w_fbase = (uint32_t)(OS / 2);
         g.w_nf = (uint32_t)(OS + 2);
         const size_t nf = g.w_nf;
@@ -799,7 +814,7 @@ int releaseLibrary(void) {
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
-                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist};
+                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;

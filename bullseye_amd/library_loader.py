@@ -31,7 +31,8 @@ class bullseye_options(ctypes.Structure):
         ("apply_taper_correction", ctypes.c_uint32),
         ("channels_per_stream", ctypes.c_uint32),
         ("rows_per_stream", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 10),
+        ("walk_order", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 9),
     ]
 
 
@@ -42,6 +43,8 @@ class bullseye_stats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double),
         ("prep_ms", ctypes.c_double),
         ("copy_ms", ctypes.c_double),
+        ("time_inner", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -199,7 +199,10 @@ typedef struct bullseye_options {
                                    taper (DFT of the sampled AA kernel) */
     uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto) */
     uint32_t rows_per_stream;   /* Romein stream: time rows walked per work item (0 = auto) */
-    uint32_t reserved[10];      /* reserved[0] = 1 forces the generic (any-support) kernel */
+    uint32_t walk_order;        /* order of a stream's samples: 0 = whichever moves the kernel
+                                   footprint less per step, measured from the data; 1 =
+                                   channels inner; 2 = time inner */
+    uint32_t reserved[9];       /* reserved[0] = 1 forces the generic (any-support) kernel */
 } bullseye_options;
 
 int bullseye_set_options(const bullseye_options *options);
@@ -222,6 +225,8 @@ typedef struct bullseye_stats {
     double kernel_ms;           /* device time of the gridding kernel(s) of the call */
     double prep_ms;             /* device time of ingest (baseline sort) of the call */
     double copy_ms;             /* device time of host->device copies of the call */
+    uint32_t time_inner;        /* walk order of the call's streams: 1 time inner, 0 channels */
+    uint32_t reserved;
 } bullseye_stats;
 int bullseye_get_stats(bullseye_stats *stats);
 

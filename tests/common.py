@@ -21,12 +21,12 @@ def mode_ncorr(mode):
 
 
 def small_obs(mode="grid_single_pol", seed=0, n_ant=12, n_time=30, n_chan=8, size_m=1000.0,
-              flag_fraction=0.05):
+              flag_fraction=0.05, f_lo=1.40e9, f_hi=1.42e9):
     n_corr = 4 if mode in MODES_4 or mode in MODES_2 else 1
     jones = mode == "facet_4_cor_corrections"
     return synth.simulate(n_ant=n_ant, n_time=n_time, n_chan=n_chan, array_size_m=size_m,
                           n_corr=n_corr, jones_sigma=0.05 if jones else None, seed=seed,
-                          flag_fraction=flag_fraction)
+                          flag_fraction=flag_fraction, f_lo=f_lo, f_hi=f_hi)
 
 
 def round_to(obs, precision):

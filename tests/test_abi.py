@@ -2,6 +2,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -64,7 +65,31 @@ def test_mixed_layout_equals_double():
 
 def test_options_and_stats_sizes():
     assert ctypes.sizeof(bullseye_options) == 64
-    assert ctypes.sizeof(bullseye_stats) == 40
+    assert ctypes.sizeof(bullseye_stats) == 48
+
+
+def test_options_and_stats_layout_match_header(tmp_path):
+    """The ctypes mirrors of the extension structs against the C header, compiled with gcc."""
+    fields = {"bullseye_options": bullseye_options, "bullseye_stats": bullseye_stats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bullseye_amd.h"',
+             'int main(void) {']
+    for name, cls in fields.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % name)
+        for f in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (name, f[0]))
+    lines.append('return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-DBULLSEYE_DOUBLE", "-I", INCLUDE_DIR, str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = []
+    for cls in fields.values():
+        want.append(ctypes.sizeof(cls))
+        want += [getattr(cls, f[0]).offset for f in cls._fields_]
+    assert got == want
 
 
 def test_loader_rejects_bad_arguments_like_the_reference():

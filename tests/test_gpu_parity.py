@@ -5,13 +5,16 @@ peak <= 1e-9; FP64 path into a float32 grid (MIXED/SINGLE builds) rms <= 2e-7, p
 output rounding); FP32 paths rms <= 1e-6 (Kahan) / 2e-6 (plain), peak <= 2e-5.  Normalisation
 terms to 1e-12 relative (FP64 sums on every path).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 from common import (ALL_MODES, MODES_PSF, make_spec, rel_err, round_to, run_library,
                     set_options, small_obs)
 from bullseye_amd.binding import Binding
-from bullseye_amd.library_loader import (PATH_FP32, PATH_FP32_KAHAN, PATH_FP64, load_library)
+from bullseye_amd.library_loader import (PATH_FP32, PATH_FP32_KAHAN, PATH_FP64, bullseye_stats,
+                                         load_library)
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -120,3 +123,55 @@ def test_singular_jones_reports_error():
         assert b"SINGULAR" in lib.bullseye_last_error()
     finally:
         lib.releaseLibrary()
+
+
+# A wide band (1.0-1.5 GHz in 8 channels) with short time steps (120 over 8 h): a channel step
+# moves the footprint about 3x further than a time step, so the measured order is time inner.
+def wideband_obs(mode, seed=4):
+    return small_obs(mode, seed=seed, n_ant=10, n_time=120, f_lo=1.0e9, f_hi=1.5e9)
+
+
+WALK_MODES = ["grid_single_pol", "grid_duel_pol", "facet_4_cor_corrections",
+              "facet_sampling_function"]
+
+
+@pytest.mark.parametrize("generic", [False, True])
+@pytest.mark.parametrize("walk", [0, 1, 2])
+@pytest.mark.parametrize("precision,path", [("double", PATH_FP64), ("mixed", PATH_FP32_KAHAN)])
+@pytest.mark.parametrize("mode", WALK_MODES)
+def test_walk_orders(mode, precision, path, walk, generic):
+    """Channels-inner, time-inner and measured walks (specialised and generic Romein kernels)
+    against the oracle; the measured order on this wide band is time inner."""
+    print("case walk", mode, precision, path, walk, generic, flush=True)
+    obs = round_to(wideband_obs(mode), precision)
+    spec = make_spec(obs, mode, precision=precision)
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", precision)
+    reserved = (ctypes.c_uint32 * 9)(1 if generic else 0)
+    set_options(lib, compute_path=path, walk_order=walk, reserved=reserved)
+    b = Binding(precision, obs, spec, with_psf=mode in MODES_PSF)
+    norm = run_library(lib, b, mode)
+    st = bullseye_stats()
+    assert lib.bullseye_get_stats(ctypes.byref(st)) == 0
+    assert st.time_inner == (0 if walk == 1 else 1)
+    got = b.psf if mode in MODES_PSF else b.output
+    rms, peak = rel_err(got, want)
+    print("rms %.3e peak %.3e" % (rms, peak))
+    tol = tolerance(precision, path)
+    assert rms <= tol[0] and peak <= tol[1], (rms, peak)
+    if mode not in MODES_PSF:
+        # FP64 sums of ~3e4 terms in another order: the bound is ~n * eps = 3e-12
+        np.testing.assert_allclose(norm, want_norm, rtol=1e-11, atol=0)
+
+
+def test_measured_walk_is_channels_inner_on_a_narrow_band():
+    """1.40-1.42 GHz in 8 channels, 30 time steps over 8 h: channels inner."""
+    obs = round_to(small_obs("grid_single_pol", seed=3), "mixed")
+    spec = make_spec(obs, "grid_single_pol", precision="mixed")
+    lib = load_library("GPU", "mixed")
+    set_options(lib, compute_path=PATH_FP32_KAHAN)
+    b = Binding("mixed", obs, spec)
+    run_library(lib, b, "grid_single_pol")
+    st = bullseye_stats()
+    assert lib.bullseye_get_stats(ctypes.byref(st)) == 0
+    assert st.time_inner == 0

@@ -19,8 +19,8 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 
-def wspec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0):
-    spec = make_spec(obs, mode, precision=precision, npix=128, support=S, oversample=OS)
+def wspec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0, npix=128):
+    spec = make_spec(obs, mode, precision=precision, npix=npix, support=S, oversample=OS)
     csz = kernels.aa_kernel_size(S, OS)
     rng = np.random.default_rng(seed)
     conv = (rng.normal(size=(planes, csz, csz)) + 1j * rng.normal(size=(planes, csz, csz))) / csz
@@ -35,14 +35,14 @@ def wspec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0):
     return spec, halves
 
 
-def run_case(precision, path, mode, halves_fn, seed=3):
+def run_case(precision, path, mode, halves_fn, seed=3, S=7, npix=128, walk=0):
     obs = round_to(small_obs(mode, seed=seed), precision)
-    spec, halves = wspec(obs, mode, precision, halves=halves_fn(8, 7))
+    spec, halves = wspec(obs, mode, precision, S=S, halves=halves_fn(8, S), npix=npix)
     ref = Binding("double", obs, spec, with_psf=mode in MODES_PSF)
     want_norm = oracle.run(ref, mode)
     want = ref.psf if mode in MODES_PSF else ref.output
     lib = load_library("GPU", precision)
-    set_options(lib, compute_path=path)
+    set_options(lib, compute_path=path, walk_order=walk)
     arr = (ctypes.c_uint32 * len(halves))(*halves)
     assert lib.bullseye_set_wplane_support(arr, len(halves)) == 0
     b = Binding(precision, obs, spec, with_psf=mode in MODES_PSF)
@@ -75,6 +75,39 @@ def test_wproj_parity(mode, precision, path, halves):
     (rms, peak), norm, want_norm, want = run_case(precision, path, mode, HALVES[halves])
     print("rms %.3e peak %.3e" % (rms, peak))
     assert np.abs(want).max() > 0
+    tol = (1e-11, 1e-9) if (precision == "double" and path == PATH_FP64) else (1e-6, 2e-5)
+    assert rms <= tol[0] and peak <= tol[1], (rms, peak)
+    if mode not in MODES_PSF:
+        np.testing.assert_allclose(norm, want_norm, rtol=1e-9)
+
+
+# Class widths 7 and 13 take the column-ownership kernel, 19..31 the residue-pair kernel with
+# one pair per lane, 39 two pairs and 47 four pairs per lane.
+WIDE = lambda n, s: [3, 6, 9, 12, 15, 19, 23, 23][:n]
+
+
+@pytest.mark.parametrize("precision,path", CASES)
+@pytest.mark.parametrize("mode", ["grid_single_pol", "grid_duel_pol", "facet_4_cor_corrections",
+                                  "grid_sampling_function"])
+def test_wproj_wide_classes(mode, precision, path):
+    print("case wproj wide", mode, precision, path, flush=True)
+    (rms, peak), norm, want_norm, want = run_case(precision, path, mode, WIDE, S=23, npix=256)
+    print("rms %.3e peak %.3e" % (rms, peak))
+    assert np.abs(want).max() > 0
+    tol = (1e-11, 1e-9) if (precision == "double" and path == PATH_FP64) else (1e-6, 2e-5)
+    assert rms <= tol[0] and peak <= tol[1], (rms, peak)
+    if mode not in MODES_PSF:
+        np.testing.assert_allclose(norm, want_norm, rtol=1e-9)
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+@pytest.mark.parametrize("precision,path", CASES)
+@pytest.mark.parametrize("mode", ["grid_single_pol", "facet_4_cor_corrections"])
+def test_wproj_walk_orders(mode, precision, path, walk):
+    print("case wproj walk", mode, precision, path, walk, flush=True)
+    (rms, peak), norm, want_norm, want = run_case(precision, path, mode, WIDE, S=23, npix=256,
+                                                  walk=walk)
+    print("rms %.3e peak %.3e" % (rms, peak))
     tol = (1e-11, 1e-9) if (precision == "double" and path == PATH_FP64) else (1e-6, 2e-5)
     assert rms <= tol[0] and peak <= tol[1], (rms, peak)
     if mode not in MODES_PSF:
