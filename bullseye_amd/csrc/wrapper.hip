@@ -33,6 +33,10 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
 int romein_max_width();
 hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
                              hipStream_t st);
+hipError_t launch_degrid(const GridArgs &a, const void *model, void *out, uint64_t rows,
+                         uint32_t P, const void *taps, const uint64_t *plane_off,
+                         const uint32_t *plane_half, uint32_t wplanes, uint32_t nf,
+                         uint32_t fbase, float wmax_f, int mode, int path, hipStream_t st);
 size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
 hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
                        uint32_t n_bl, uint32_t *keys, uint32_t *vals, uint32_t *keys_out,
@@ -99,7 +103,7 @@ struct State {
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
-    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk;
+    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
     DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
@@ -299,48 +303,15 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
     }
 }
 
-static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
-    if (!g.initialized) return fail("initLibrary has not been called");
-    const bool psf = mode == MODE_PSF;
-    const bool wproj = g.wplanes > 1;
-    if ((p->wplanes > 1) != wproj) return fail("wplanes differs from initLibrary");
-    const int W = (int)(2 * p->conv_support + 1);
-    if (!wproj && W > romein_max_width())
-        return fail("conv_support %zu too large for the AA gridder (max full support %d)",
-                    (size_t)p->conv_support, romein_max_width());
-    if (p->conv_support != g.S || p->conv_oversample != g.OS)
-        return fail("conv_support/conv_oversample differ from initLibrary");
-    const int ncorr = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
-    if (!psf && (size_t)ncorr > g.P)
-        return fail("mode grids %d correlations but number_of_polarization_terms_being_gridded is %zu",
-                    ncorr, g.P);
-    if (psf && !g.psf) return fail("sampling function buffer was not set up at initLibrary");
-    if (!p->antenna_1_ids || !p->antenna_2_ids)
-        return fail("antenna_1_ids/antenna_2_ids are required (Romein gridding walks baselines)");
-    if (mode == MODE_4J && !p->jones_terms) return fail("jones_terms is NULL");
-    if (p->row_count > 0xffffffffull) return fail("row_count too large");
+// Inputs of a gridding or degridding call on the device (staged copies, or the caller's own
+// device pointers), the inverted Jones terms, the per-facet geometry and the scalar fields of
+// GridArgs that do not depend on the stream decomposition.
+static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase, bool psf,
+                        bool need_vis, GridArgs &a) {
     const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
     const size_t nspw = std::max<size_t>(1, p->spw_count);
-    const size_t B = p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
-    if (mode == MODE_1) {
-        if (p->polarization_index >= npol) return fail("polarization_index out of range");
-    } else if (mode == MODE_2) {
-        if (p->polarization_index >= npol || p->second_polarization_index >= npol)
-            return fail("polarization indexes out of range");
-    } else if ((mode == MODE_4 || mode == MODE_4J) && npol < 4) {
-        return fail("4-correlation gridding needs 4 correlations per sample");
-    }
-
-    TimedCall tc{};
-    HIP_TRY(hipEventCreate(&tc.start));
-    HIP_TRY(hipEventCreate(&tc.k0));
-    HIP_TRY(hipEventCreate(&tc.k1));
-    HIP_TRY(hipEventCreate(&tc.stop));
-    HIP_TRY(hipEventRecord(tc.start, g.stream));
-
-    GridArgs a{};
     const void *d;
-    HIP_TRY(stage(g.in_vis, psf ? nullptr : p->visibilities,
+    HIP_TRY(stage(g.in_vis, (psf || !need_vis) ? nullptr : p->visibilities,
                   rows * C * npol * 2 * sizeof(visibility_base_type), &d));
     a.vis = (const visibility_base_type *)d;
     HIP_TRY(stage(g.in_uvw, p->uvw_coords, rows * 3 * sizeof(uvw_base_type), &d));
@@ -370,7 +341,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
                   nspw * C * sizeof(size_t), &d));
     a.cgi = (const uint64_t *)d;
     if (!a.uvw || !a.wavelengths || !a.row_flags || !a.field || !a.spw || !a.enabled || !a.cgi ||
-        (!psf && (!a.vis || !a.weights || !a.flags)))
+        (!psf && ((need_vis && !a.vis) || !a.weights || !a.flags)))
         return fail("a required array pointer is NULL");
     if (mode == MODE_4J) {
         HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
@@ -413,6 +384,92 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     HIP_TRY(g.geom.ensure(g.F * sizeof(FacetGeom)));
     HIP_TRY(hipMemcpyAsync(g.geom.p, geo.data(), g.F * sizeof(FacetGeom), hipMemcpyHostToDevice,
                            g.stream));
+    a.geom = g.geom.as<FacetGeom>();
+    a.conv = g.path == PATH_FP64 ? g.conv_d.p : g.conv_f.p;
+    a.tapsum = g.tapsum.as<double>();
+    a.error_flag = g.errflag.as<int>();
+    a.nx = (uint32_t)g.nx;
+    a.ny = (uint32_t)g.ny;
+    a.S = (uint32_t)g.S;
+    a.W = (uint32_t)(2 * p->conv_support + 1);
+    a.OS = (uint32_t)g.OS;
+    a.csz = (uint32_t)g.csz;
+    a.chan = (uint32_t)C;
+    a.npol = (uint32_t)npol;
+    a.spw_count = (uint32_t)nspw;
+    a.antenna_count = (uint32_t)p->antenna_count;
+    a.ntime = (uint32_t)p->no_timestamps_read;
+    a.facets = (uint32_t)g.F;
+    a.cube = (uint32_t)(psf ? g.sfC : g.cube);
+    a.pstride = (uint32_t)(psf ? 1 : g.pstride);
+    for (int k = 0; k < 4; ++k) a.pol_idx[k] = (uint32_t)k;
+    if (mode == MODE_1) a.pol_idx[0] = (uint32_t)p->polarization_index;
+    if (mode == MODE_2) {
+        a.pol_idx[0] = (uint32_t)p->polarization_index;
+        a.pol_idx[1] = (uint32_t)p->second_polarization_index;
+    }
+    a.imaging_field = p->imaging_field;
+    // templated_gridder.h:23-27 (u_scale uses the float ARCSEC_TO_RAD constant)
+    a.u_scale = (double)((double)p->nx * (double)p->cell_size_x * (double)ARCSEC_TO_RAD);
+    a.v_scale = -(double)((double)p->ny * (double)p->cell_size_y * (double)ARCSEC_TO_RAD);
+    a.off_x = (double)(p->nx / 2 - p->conv_support);
+    a.off_y = (double)(p->ny / 2 - p->conv_support);
+    a.phase = phase ? 1 : 0;
+    a.transform = facet ? 1 : 0;
+    return 0;
+}
+
+// Checks shared by the gridding and degridding entry points.
+static int validate_call(const gridding_parameters *p, int mode) {
+    const bool psf = mode == MODE_PSF;
+    const bool wproj = g.wplanes > 1;
+    if ((p->wplanes > 1) != wproj) return fail("wplanes differs from initLibrary");
+    const int W = (int)(2 * p->conv_support + 1);
+    if (!wproj && W > romein_max_width())
+        return fail("conv_support %zu too large for the AA gridder (max full support %d)",
+                    (size_t)p->conv_support, romein_max_width());
+    if (p->conv_support != g.S || p->conv_oversample != g.OS)
+        return fail("conv_support/conv_oversample differ from initLibrary");
+    const int ncorr = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
+    if (!psf && (size_t)ncorr > g.P)
+        return fail("mode grids %d correlations but number_of_polarization_terms_being_gridded is %zu",
+                    ncorr, g.P);
+    if (psf && !g.psf) return fail("sampling function buffer was not set up at initLibrary");
+    if (!p->antenna_1_ids || !p->antenna_2_ids)
+        return fail("antenna_1_ids/antenna_2_ids are required (Romein gridding walks baselines)");
+    if (mode == MODE_4J && !p->jones_terms) return fail("jones_terms is NULL");
+    if (p->row_count > 0xffffffffull) return fail("row_count too large");
+    const size_t npol = p->number_of_polarization_terms;
+    if (mode == MODE_1) {
+        if (p->polarization_index >= npol) return fail("polarization_index out of range");
+    } else if (mode == MODE_2) {
+        if (p->polarization_index >= npol || p->second_polarization_index >= npol)
+            return fail("polarization indexes out of range");
+    } else if ((mode == MODE_4 || mode == MODE_4J) && npol < 4) {
+        return fail("4-correlation gridding needs 4 correlations per sample");
+    }
+    return 0;
+}
+
+static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    const bool psf = mode == MODE_PSF;
+    const bool wproj = g.wplanes > 1;
+    if (validate_call(p, mode)) return 1;
+    const int W = (int)(2 * p->conv_support + 1);
+    const size_t rows = p->row_count, C = p->channel_count;
+    const size_t nspw = std::max<size_t>(1, p->spw_count);
+    const size_t B = p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
+
+    TimedCall tc{};
+    HIP_TRY(hipEventCreate(&tc.start));
+    HIP_TRY(hipEventCreate(&tc.k0));
+    HIP_TRY(hipEventCreate(&tc.k1));
+    HIP_TRY(hipEventCreate(&tc.stop));
+    HIP_TRY(hipEventRecord(tc.start, g.stream));
+
+    GridArgs a{};
+    if (setup_inputs(p, mode, facet, phase, psf, true, a)) return 1;
 
     // ingest: stable sort of rows by baseline
     HIP_TRY(g.perm.ensure(rows * 4));
@@ -447,48 +504,16 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
 
     a.perm = g.perm.as<uint32_t>();
     a.bl_start = g.blstart.as<uint32_t>();
-    a.geom = g.geom.as<FacetGeom>();
-    a.conv = g.path == PATH_FP64 ? g.conv_d.p : g.conv_f.p;
-    a.tapsum = g.tapsum.as<double>();
     a.grid = psf ? g.psfgrid.as<double>() : g.grid.as<double>();
     a.norm = g.norm.as<double>();
     a.counters = g.counters.as<unsigned long long>();
     a.norm_stride = g.norm_stride;
     a.nslots = g.nslots;
-    a.error_flag = g.errflag.as<int>();
-    a.nx = (uint32_t)g.nx;
-    a.ny = (uint32_t)g.ny;
-    a.S = (uint32_t)g.S;
-    a.W = (uint32_t)W;
-    a.OS = (uint32_t)g.OS;
-    a.csz = (uint32_t)g.csz;
-    a.chan = (uint32_t)C;
-    a.npol = (uint32_t)npol;
-    a.spw_count = (uint32_t)nspw;
-    a.antenna_count = (uint32_t)p->antenna_count;
-    a.ntime = (uint32_t)p->no_timestamps_read;
-    a.facets = (uint32_t)g.F;
-    a.cube = (uint32_t)(psf ? g.sfC : g.cube);
-    a.pstride = (uint32_t)(psf ? 1 : g.pstride);
-    for (int k = 0; k < 4; ++k) a.pol_idx[k] = (uint32_t)k;
-    if (mode == MODE_1) a.pol_idx[0] = (uint32_t)p->polarization_index;
-    if (mode == MODE_2) {
-        a.pol_idx[0] = (uint32_t)p->polarization_index;
-        a.pol_idx[1] = (uint32_t)p->second_polarization_index;
-    }
-    a.imaging_field = p->imaging_field;
     a.baselines = (uint32_t)B;
     a.chan_per_stream = cps;
     a.n_chan_blocks = (uint32_t)((C + cps - 1) / cps);
     a.n_segments = nseg;
     a.n_streams = (uint64_t)B * a.n_chan_blocks * nseg * g.F;
-    // templated_gridder.h:23-27 (u_scale uses the float ARCSEC_TO_RAD constant)
-    a.u_scale = (double)((double)p->nx * (double)p->cell_size_x * (double)ARCSEC_TO_RAD);
-    a.v_scale = -(double)((double)p->ny * (double)p->cell_size_y * (double)ARCSEC_TO_RAD);
-    a.off_x = (double)(p->nx / 2 - p->conv_support);
-    a.off_y = (double)(p->ny / 2 - p->conv_support);
-    a.phase = phase ? 1 : 0;
-    a.transform = facet ? 1 : 0;
     HIP_TRY(g.walk.ensure(sizeof(int)));
     a.time_inner = g.walk.as<int>();
     HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, g.walk.as<int>(), g.stream));
@@ -814,7 +839,8 @@ int releaseLibrary(void) {
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
-                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk};
+                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
+                      &g.dg_model, &g.dg_out};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
@@ -984,5 +1010,53 @@ int facet_4_cor_corrections(gridding_parameters *p) { return grid_call(p, MODE_4
 int grid_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, false, false); }
 // wrapper.cpp:402-425: faceting uvw transform but no phase shift for the sampling function
 int facet_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, true, false); }
+
+// The degridding adjoint of a gridding entry point (visibility modes).  Synchronous.
+int bullseye_degrid(gridding_parameters *p, int mode, const void *model, void *vis_out) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (!p || !model || !vis_out) return fail("bullseye_degrid: NULL argument");
+    static const struct { int kmode; bool facet, phase; } modes[] = {
+        {MODE_1, false, false}, {MODE_1, true, true}, {MODE_2, false, false},
+        {MODE_2, true, true},   {MODE_4, false, false}, {MODE_4, true, true},
+        {MODE_4J, true, true}};
+    if (mode < 0 || mode > 6) return fail("bullseye_degrid: mode %d is not a visibility mode", mode);
+    const int km = modes[mode].kmode;
+    if (validate_call(p, km)) return 1;
+    if (barrier_impl()) return 1;
+    const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
+    const size_t model_bytes = g.F * g.cube * g.P * g.ny * g.nx * 2 * sizeof(grid_base_type);
+    const size_t out_bytes = rows * C * npol * 2 * sizeof(visibility_base_type);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    GridArgs a{};
+    if (setup_inputs(p, km, modes[mode].facet, modes[mode].phase, false, false, a)) return 1;
+    const void *dm = model;
+    void *dout = vis_out;
+    if (!dev_mode()) {
+        HIP_TRY(g.dg_model.ensure(model_bytes));
+        HIP_TRY(g.dg_out.ensure(std::max<size_t>(out_bytes, 16)));
+        HIP_TRY(hipMemcpyAsync(g.dg_model.p, model, model_bytes, hipMemcpyHostToDevice, g.stream));
+        dm = g.dg_model.p;
+        dout = g.dg_out.p;
+    }
+    const bool wproj = g.wplanes > 1;
+    const void *taps = wproj ? (g.path == PATH_FP64 ? g.wtaps_d.p : g.wtaps_f.p) : nullptr;
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    HIP_TRY(launch_degrid(a, dm, dout, rows, (uint32_t)g.pstride, taps,
+                          g.wplane_off.as<uint64_t>(), g.wplane_half.as<uint32_t>(),
+                          (uint32_t)g.wplanes, g.w_nf, g.w_fbase, (float)p->wmax_est, km, g.path,
+                          g.stream));
+    HIP_TRY(hipEventRecord(e1, g.stream));
+    if (!dev_mode())
+        HIP_TRY(hipMemcpyAsync(vis_out, dout, out_bytes, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g.stats.kernel_ms = ms;
+    return check_device_errors();
+}
 
 }  // extern "C"

@@ -20,7 +20,8 @@ ENTRY_POINTS = (
 ALL_SYMBOLS = ENTRY_POINTS + ("get_gridding_walltime", "get_inversion_walltime",
                               "gridding_barrier")
 EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_get_options",
-                     "bullseye_reset_grids", "bullseye_get_stats", "bullseye_set_wplane_support")
+                     "bullseye_reset_grids", "bullseye_get_stats", "bullseye_set_wplane_support",
+                     "bullseye_degrid")
 
 
 class bullseye_options(ctypes.Structure):
@@ -86,4 +87,6 @@ def load_library(architecture, precision, share_torch_runtime=True):
     lib.bullseye_set_options.argtypes = [ctypes.POINTER(bullseye_options)]
     lib.bullseye_get_options.argtypes = [ctypes.POINTER(bullseye_options)]
     lib.bullseye_get_stats.argtypes = [ctypes.POINTER(bullseye_stats)]
+    lib.bullseye_degrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p]
     return lib

@@ -230,6 +230,32 @@ typedef struct bullseye_stats {
 } bullseye_stats;
 int bullseye_get_stats(bullseye_stats *stats);
 
+/* Gridding modes, numbered in the order of the entry points above. */
+enum bullseye_grid_mode {
+    BULLSEYE_GRID_SINGLE_POL = 0,
+    BULLSEYE_FACET_SINGLE_POL = 1,
+    BULLSEYE_GRID_DUEL_POL = 2,
+    BULLSEYE_FACET_DUEL_POL = 3,
+    BULLSEYE_GRID_4_COR = 4,
+    BULLSEYE_FACET_4_COR = 5,
+    BULLSEYE_FACET_4_COR_CORRECTIONS = 6,
+    BULLSEYE_GRID_SAMPLING_FUNCTION = 7,
+    BULLSEYE_FACET_SAMPLING_FUNCTION = 8
+};
+
+/* Degridding: the adjoint of a gridding entry point.  No counterpart in bullseye's C++ library
+ * (writing_and_wiki/notebooks/Resampling.ipynb sketches the gather in Python).  mode is one of
+ * the visibility modes 0..6.  model has output_buffer's layout (complex [F][cube][P][ny][nx],
+ * grid type); vis_out has the visibilities' layout (complex [rows][chan][npol], visibility
+ * type) and is overwritten.  Each gridded correlation receives the transpose, over the reals,
+ * of the gridding map applied to model: conjugated kernel taps, the w < 0 conjugation of
+ * w-projection, the conjugate facet phase, weights and flags, and Pinv^H S Qinv for the Jones
+ * mode; no normalisation.  Every other correlation is zero.  Uses the struct's uvw,
+ * wavelengths, weights, flags, facets and Jones and the kernels given to initLibrary, on the
+ * compute path of bullseye_options.  Synchronous; with device_resident, model and vis_out are
+ * device pointers. */
+int bullseye_degrid(gridding_parameters *params, int mode, const void *model, void *vis_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -119,6 +119,42 @@ int oracle_invert_all(vis_c *jones, size_t count) {
     return 0;
 }
 
+/* Per-facet geometry: templated_gridder.h:36-42 (facet centre), baseline_transform_policies.h:
+ * 120-143 (planar uvw transform with w) and phase_transform_policies.h:110-134 (phase offset). */
+typedef struct {
+    uvw_base_type u_term, v_term, dl, dm, dn;
+} facet_geom;
+
+static facet_geom facet_setup(const gridding_parameters *p, size_t f, int facet, int phase) {
+    facet_geom g = {0, 0, 0, 0, 0};
+    uvw_base_type ra = p->phase_centre_ra, dec = p->phase_centre_dec;
+    if (facet) {
+        ra = p->facet_centres[2 * f];
+        dec = p->facet_centres[2 * f + 1];
+    }
+    if (facet) {
+        uvw_base_type d_ra = (ra - p->phase_centre_ra) * ARCSEC_TO_RAD,
+                      n_dec = dec * ARCSEC_TO_RAD, o_dec = p->phase_centre_dec * ARCSEC_TO_RAD,
+                      c_d_ra = cos(d_ra), s_d_ra = sin(d_ra), c_new_dec = cos(n_dec),
+                      c_old_dec = cos(o_dec), s_new_dec = sin(n_dec), s_old_dec = sin(o_dec),
+                      li0 = c_new_dec * s_d_ra,
+                      mi0 = s_new_dec * c_old_dec - c_new_dec * s_old_dec * c_d_ra,
+                      ni0 = s_new_dec * s_old_dec + c_new_dec * c_old_dec * c_d_ra;
+        g.u_term = li0 / ni0;
+        g.v_term = mi0 / ni0;
+    }
+    if (phase) {
+        uvw_base_type d_ra = (ra - p->phase_centre_ra) * ARCSEC_TO_RAD,
+                      d_dec = dec * ARCSEC_TO_RAD, d_decp = p->phase_centre_dec * ARCSEC_TO_RAD,
+                      c_d_dec = cos(d_dec), s_d_dec = sin(d_dec), s_d_ra = sin(d_ra),
+                      c_d_ra = cos(d_ra), c_d_decp = cos(d_decp), s_d_decp = sin(d_decp);
+        g.dl = -c_d_dec * s_d_ra;
+        g.dm = -(s_d_dec * c_d_decp - c_d_dec * s_d_decp * c_d_ra);
+        g.dn = 1 - sqrt(1 - g.dl * g.dl - g.dm * g.dm);
+    }
+    return g;
+}
+
 /* One facet of cpu_algorithm/templated_gridder.h:12-104 with the policies chosen by
  * cpu_algorithm/wrapper.cpp:182-425 (mode) and wplanes (AA 1D vs 2D w-projection). */
 static void grid_facet(const gridding_parameters *p, int mode, size_t f) {
@@ -148,33 +184,9 @@ static void grid_facet(const gridding_parameters *p, int mode, size_t f) {
                                    slice * p->sampling_function_channel_count * f
                              : (grid_c *)p->output_buffer + slice * P * p->cube_channel_dim_size * f;
 
-    /* templated_gridder.h:36-42: facet centre, uvw transform and phase offset */
-    uvw_base_type ra = p->phase_centre_ra, dec = p->phase_centre_dec;
-    if (facet) {
-        ra = p->facet_centres[2 * f];
-        dec = p->facet_centres[2 * f + 1];
-    }
-    uvw_base_type u_term = 0, v_term = 0, dl = 0, dm = 0, dn = 0;
-    if (facet) { /* baseline_transform_policies.h:120-143 (planar approximation with w) */
-        uvw_base_type d_ra = (ra - p->phase_centre_ra) * ARCSEC_TO_RAD,
-                      n_dec = dec * ARCSEC_TO_RAD, o_dec = p->phase_centre_dec * ARCSEC_TO_RAD,
-                      c_d_ra = cos(d_ra), s_d_ra = sin(d_ra), c_new_dec = cos(n_dec),
-                      c_old_dec = cos(o_dec), s_new_dec = sin(n_dec), s_old_dec = sin(o_dec),
-                      li0 = c_new_dec * s_d_ra,
-                      mi0 = s_new_dec * c_old_dec - c_new_dec * s_old_dec * c_d_ra,
-                      ni0 = s_new_dec * s_old_dec + c_new_dec * c_old_dec * c_d_ra;
-        u_term = li0 / ni0;
-        v_term = mi0 / ni0;
-    }
-    if (phase) { /* phase_transform_policies.h:110-134 */
-        uvw_base_type d_ra = (ra - p->phase_centre_ra) * ARCSEC_TO_RAD,
-                      d_dec = dec * ARCSEC_TO_RAD, d_decp = p->phase_centre_dec * ARCSEC_TO_RAD,
-                      c_d_dec = cos(d_dec), s_d_dec = sin(d_dec), s_d_ra = sin(d_ra),
-                      c_d_ra = cos(d_ra), c_d_decp = cos(d_decp), s_d_decp = sin(d_decp);
-        dl = -c_d_dec * s_d_ra;
-        dm = -(s_d_dec * c_d_decp - c_d_dec * s_d_decp * c_d_ra);
-        dn = 1 - sqrt(1 - dl * dl - dm * dm);
-    }
+    const facet_geom fg = facet_setup(p, f, facet, phase);
+    const uvw_base_type u_term = fg.u_term, v_term = fg.v_term, dl = fg.dl, dm = fg.dm,
+                        dn = fg.dn;
     const size_t conv_dim = padded + (padded - 1) * (OS - 1);
     const convolution_base_type *conv_r = p->conv;
     const conv_c *conv_w = (const conv_c *)p->conv;
@@ -358,6 +370,182 @@ int oracle_grid(gridding_parameters *p, int mode) {
 #pragma omp parallel for schedule(static)
     for (size_t f = 0; f < p->num_facet_centres; ++f) grid_facet(p, mode, f);
     g_gridding_seconds += now_seconds() - t0;
+    return 0;
+}
+
+/* The degridding adjoint (no reference code exists; bullseye's Resampling.ipynb cell 17 is a
+ * Python sketch of the gather).  For the visibility modes (not PSF) this is the exact
+ * transpose, over the reals, of grid_facet's map from visibilities to the unnormalised grids,
+ * summed over facets: per (row, channel) and facet, with the forward's geometry, tap choice,
+ * edge drop and plane choice,
+ *   s_k = sum over the footprint of conj(C) * model_k       (C real for the AA kernel)
+ *   s_k = conj(s_k) when the w-projection sample had w < 0  (the forward conjugated it)
+ *   s_k *= exp(-i x) (facet phase), s_k *= weight_k * keep_k,
+ *   S = Pinv^H S Qinv for the Jones mode (the forward is V -> Pinv V Qinv^H),
+ * and out[row][chan][corr_k] = sum over facets of s_k; every other correlation is zero.
+ * model has output_buffer's layout; out has the visibilities' layout.  The Jones terms are
+ * inverted on a private copy. */
+int oracle_degrid(const gridding_parameters *p, int mode, const grid_c *model, vis_c *out) {
+    if (mode < GRID_SINGLE || mode > FACET_4_JONES) {
+        snprintf(g_error, sizeof g_error, "degridding is defined for the visibility modes only");
+        return 1;
+    }
+    const int wproj = p->wplanes > 1;
+    const int facet = (mode == FACET_SINGLE || mode == FACET_DUEL || mode == FACET_4 ||
+                       mode == FACET_4_JONES);
+    const int phase = facet;
+    const int jones = (mode == FACET_4_JONES);
+    size_t ncorr = 1;
+    if (mode == GRID_DUEL || mode == FACET_DUEL) ncorr = 2;
+    if (mode == GRID_4 || mode == FACET_4 || mode == FACET_4_JONES) ncorr = 4;
+    const size_t P = p->number_of_polarization_terms_being_gridded;
+    const size_t F = p->num_facet_centres, npol = p->number_of_polarization_terms;
+    const size_t S = p->conv_support, OS = p->conv_oversample;
+    const size_t full = 2 * S + 1, padded = full + 2;
+    const size_t slice = p->nx * p->ny;
+    const size_t conv_dim = padded + (padded - 1) * (OS - 1);
+    vis_c *jinv = NULL;
+    if (jones) {
+        size_t n = p->no_timestamps_read * p->antenna_count * F * p->spw_count * p->channel_count;
+        jinv = (vis_c *)malloc(4 * n * sizeof(vis_c));
+        memcpy(jinv, p->jones_terms, 4 * n * sizeof(vis_c));
+        if (oracle_invert_all(jinv, n)) {
+            free(jinv);
+            return 1;
+        }
+    }
+    facet_geom *geo = (facet_geom *)malloc((F ? F : 1) * sizeof(facet_geom));
+    for (size_t f = 0; f < F; ++f) geo[f] = facet_setup(p, f, facet, phase);
+    memset(out, 0, p->row_count * p->channel_count * npol * sizeof(vis_c));
+    const convolution_base_type *conv_r = p->conv;
+    const conv_c *conv_w = (const conv_c *)p->conv;
+
+#pragma omp parallel for schedule(static)
+    for (size_t row = 0; row < p->row_count; ++row) {
+        fesetround(wproj ? FE_TONEAREST : FE_DOWNWARD);
+        const uvw_base_type u_scale = p->nx * p->cell_size_x * ARCSEC_TO_RAD;
+        const uvw_base_type v_scale = -(p->ny * p->cell_size_y * ARCSEC_TO_RAD);
+        const uvw_base_type off_x = p->nx / 2 - p->conv_support;
+        const uvw_base_type off_y = p->ny / 2 - p->conv_support;
+        const uvw_base_type *uvw = p->uvw_coords + 3 * row;
+        const int row_flagged = p->flagged_rows[row];
+        const int in_field = p->field_array[row] == p->imaging_field;
+        const size_t spw = p->spw_index_array[row];
+        for (size_t c = 0; c < p->channel_count; ++c) {
+            const size_t fc = spw * p->channel_count + c;
+            if (!p->enabled_channels[fc]) continue;
+            const size_t cgi = p->channel_grid_indicies[fc];
+            reference_wavelengths_base_type rw = 1 / p->reference_wavelengths[fc];
+            const size_t base = (row * p->channel_count + c) * npol;
+            size_t corr[4];
+            visibility_weights_base_type cw[4];
+            for (size_t k = 0; k < ncorr; ++k) {
+                corr[k] = k;
+                if (ncorr == 1) corr[k] = p->polarization_index;
+                if (ncorr == 2) corr[k] = k == 0 ? p->polarization_index
+                                                 : p->second_polarization_index;
+                int keep = !(p->flags[base + corr[k]] || row_flagged) && in_field;
+                cw[k] = p->visibility_weights[base + corr[k]] * (visibility_base_type)keep;
+            }
+            vis_c acc[4] = {0, 0, 0, 0};
+            for (size_t f = 0; f < F; ++f) {
+                uvw_base_type u = uvw[0] * rw, v = uvw[1] * rw, w = uvw[2] * rw;
+                uvw_base_type s_ = 0, c_ = 1;
+                if (phase) {
+                    uvw_base_type x = 2 * M_PI * (u * geo[f].dl + v * geo[f].dm + w * geo[f].dn);
+#ifdef BULLSEYE_SINGLE
+                    sincosf(x, &s_, &c_);
+#else
+                    sincos(x, &s_, &c_);
+#endif
+                }
+                if (facet) {
+                    u = u - w * geo[f].u_term;
+                    v = v - w * geo[f].v_term;
+                }
+                u *= u_scale;
+                v *= v_scale;
+                const grid_c *g = model + (f * p->cube_channel_dim_size + cgi) * P * slice;
+                vis_c sk[4] = {0, 0, 0, 0};
+                int conj_out = 0;
+                if (!wproj) {
+                    uvw_base_type tu = u + off_x, tv = v + off_y;
+                    size_t disc_u = (size_t)lrint(tu), disc_v = (size_t)lrint(tv);
+                    size_t frac_u = (size_t)((1 - u + lrint(u)) * OS);
+                    size_t frac_v = (size_t)((1 - v + lrint(v)) * OS);
+                    if (disc_v + padded >= p->ny || disc_u + padded >= p->nx ||
+                        disc_v >= p->ny || disc_u >= p->nx)
+                        continue;
+                    size_t cv_i = frac_v;
+                    for (size_t sv = 0; sv < full; ++sv) {
+                        convolution_base_type cvw = conv_r[cv_i];
+                        size_t cu_i = frac_u;
+                        for (size_t su = 0; su < full; ++su) {
+                            convolution_base_type cww = conv_r[cu_i] * cvw;
+                            size_t pos = (disc_v + sv) * p->nx + disc_u + su;
+                            for (size_t k = 0; k < ncorr; ++k) sk[k] += cww * g[k * slice + pos];
+                            cu_i += OS;
+                        }
+                        cv_i += OS;
+                    }
+                } else {
+                    if (w < 0) {
+                        conj_out = 1;
+                        u *= -1;
+                        v *= -1;
+                        w *= -1;
+                    }
+                    uvw_base_type tu = u + off_x, tv = v + off_y;
+                    size_t disc_u = (size_t)lrint(tu), disc_v = (size_t)lrint(tv);
+                    size_t frac_u = (size_t)((1 - u + lrint(u)) * OS);
+                    size_t frac_v = (size_t)((1 - v + lrint(v)) * OS);
+                    size_t plane = (size_t)lrint(fabs(w) / (float)p->wmax_est * (p->wplanes - 1));
+                    if (disc_v + padded >= p->ny || disc_u + padded >= p->nx ||
+                        disc_v >= p->ny || disc_u >= p->nx || plane >= p->wplanes)
+                        continue;
+                    size_t cv_i = plane * conv_dim * conv_dim + frac_v * conv_dim;
+                    for (size_t sv = 0; sv < full; ++sv) {
+                        size_t cu_i = frac_u;
+                        for (size_t su = 0; su < full; ++su) {
+                            conv_c cww = CONJ(conv_w[cv_i + cu_i]);
+                            size_t pos = (disc_v + sv) * p->nx + disc_u + su;
+                            for (size_t k = 0; k < ncorr; ++k) sk[k] += cww * g[k * slice + pos];
+                            cu_i += OS;
+                        }
+                        cv_i += OS * conv_dim;
+                    }
+                }
+                for (size_t k = 0; k < ncorr; ++k) {
+                    if (conj_out) sk[k] = CONJ(sk[k]);
+                    sk[k] *= MKC(c_, -s_);
+                    sk[k] *= cw[k];
+                }
+                if (jones) {
+                    size_t t = p->timestamp_ids[row];
+                    size_t a1 = p->antenna_1_ids[row], a2 = p->antenna_2_ids[row];
+                    size_t j1 = (((t * p->antenna_count + a1) * F + f) * p->spw_count + spw) *
+                                    p->channel_count + c;
+                    size_t j2 = (((t * p->antenna_count + a2) * F + f) * p->spw_count + spw) *
+                                    p->channel_count + c;
+                    const vis_c *A = jinv + 4 * j1, *B = jinv + 4 * j2;
+                    /* T = A^H S, then T B */
+                    vis_c t0 = CONJ(A[0]) * sk[0] + CONJ(A[2]) * sk[2];
+                    vis_c t1 = CONJ(A[0]) * sk[1] + CONJ(A[2]) * sk[3];
+                    vis_c t2 = CONJ(A[1]) * sk[0] + CONJ(A[3]) * sk[2];
+                    vis_c t3 = CONJ(A[1]) * sk[1] + CONJ(A[3]) * sk[3];
+                    sk[0] = t0 * B[0] + t1 * B[2];
+                    sk[1] = t0 * B[1] + t1 * B[3];
+                    sk[2] = t2 * B[0] + t3 * B[2];
+                    sk[3] = t2 * B[1] + t3 * B[3];
+                }
+                for (size_t k = 0; k < ncorr; ++k) acc[k] += sk[k];
+            }
+            for (size_t k = 0; k < ncorr; ++k) out[base + corr[k]] = acc[k];
+        }
+        fesetround(FE_TONEAREST);
+    }
+    free(geo);
+    free(jinv);
     return 0;
 }
 

@@ -65,6 +65,26 @@ def run(binding, mode, precision="double", normalize=True, weight_uniformly=Fals
     return norm
 
 
+def degrid(binding, mode, model, precision="double"):
+    """The oracle's degridding adjoint of `mode` on binding's inputs (uvw, weights, flags,
+    Jones, facets, kernel): model has output_buffer's shape [F, cube, P, ny, nx]; returns the
+    complex visibilities [rows, chan, npol]."""
+    lib = load(precision)
+    p = binding.params
+    dt = np.complex128 if precision == "double" else np.complex64
+    m = np.ascontiguousarray(model, dtype=dt)
+    want = (p.num_facet_centres * p.cube_channel_dim_size *
+            p.number_of_polarization_terms_being_gridded * p.ny * p.nx)
+    if m.size != want:
+        raise ValueError("model has %d elements, the grids %d" % (m.size, want))
+    out = np.zeros((p.row_count, p.channel_count, p.number_of_polarization_terms), dtype=dt)
+    rc = lib.oracle_degrid(binding.byref(), MODES[mode], m.ctypes.data_as(ctypes.c_void_p),
+                           out.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return out
+
+
 def baseline_starts(ant1, ant2, n_ant):
     """Prefix scan of rows per baseline and the stable per-baseline row order
     (gpu_algorithm/wrapper.cu:358-454)."""

@@ -131,3 +131,21 @@ def tiny_obs(uvw, vis, n_ant=4, wavelength=1.0, n_corr=1, ant=None):
 
 def u_scale(npix, cell):
     return npix * cell * float(np.float32(np.pi / (180.0 * 3600.0)))
+
+
+def wproj_spec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0, npix=128):
+    """W-projection spec with a random complex kernel stack truncated per plane (no symmetry
+    hides a transposed or mirrored tap); wmax puts the largest |w| beyond the last plane."""
+    spec = make_spec(obs, mode, precision=precision, npix=npix, support=S, oversample=OS)
+    csz = kernels.aa_kernel_size(S, OS)
+    rng = np.random.default_rng(seed)
+    conv = (rng.normal(size=(planes, csz, csz)) + 1j * rng.normal(size=(planes, csz, csz))) / csz
+    if halves is None:
+        halves = [S] * planes
+    conv = kernels.truncate_wkernels(conv, halves, S, OS)
+    bt = force_precision(precision)
+    spec.conv = conv.astype(bt.w_fir_type).astype(np.complex128)
+    spec.wplanes = planes
+    w = np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :])
+    spec.wmax = float(np.float32(w.max() * 0.9))
+    return spec, halves

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from common import (ALL_MODES, MODES_PSF, make_spec, rel_err, round_to, set_options,
-                    small_obs)
+                    small_obs, wproj_spec)
 from bullseye_amd import kernels
 from bullseye_amd.base_types import force_precision
 from bullseye_amd.binding import Binding
@@ -20,19 +20,8 @@ pytestmark = pytest.mark.gpu
 
 
 def wspec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0, npix=128):
-    spec = make_spec(obs, mode, precision=precision, npix=npix, support=S, oversample=OS)
-    csz = kernels.aa_kernel_size(S, OS)
-    rng = np.random.default_rng(seed)
-    conv = (rng.normal(size=(planes, csz, csz)) + 1j * rng.normal(size=(planes, csz, csz))) / csz
-    if halves is None:
-        halves = [S] * planes
-    conv = kernels.truncate_wkernels(conv, halves, S, OS)
-    bt = force_precision(precision)
-    spec.conv = conv.astype(bt.w_fir_type).astype(np.complex128)
-    spec.wplanes = planes
-    w = np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :])
-    spec.wmax = float(np.float32(w.max() * 0.9))   # the largest |w| fall beyond the last plane
-    return spec, halves
+    return wproj_spec(obs, mode, precision, S=S, OS=OS, planes=planes, halves=halves,
+                      seed=seed, npix=npix)
 
 
 def run_case(precision, path, mode, halves_fn, seed=3, S=7, npix=128, walk=0):
