@@ -183,6 +183,17 @@ def simulate(n_ant=64, array="disc", array_size_m=3000.0, n_time=240, hours=8.0,
         meta=dict(seed=seed, array=array, array_size_m=array_size_m, hours=hours))
 
 
+def select_rows(obs, rows):
+    """The observation restricted to the given row indexes (order kept)."""
+    import dataclasses
+    r = np.asarray(rows)
+    return dataclasses.replace(
+        obs, ant1=obs.ant1[r], ant2=obs.ant2[r], time_idx=obs.time_idx[r], uvw=obs.uvw[r],
+        vis=obs.vis[r], weights=obs.weights[r], flags=obs.flags[r],
+        flagged_rows=obs.flagged_rows[r], field=obs.field[r], spw=obs.spw[r],
+        meta=dict(obs.meta, rows_selected=int(r.size)))
+
+
 def facet_centres(n_facets, radius_deg, phase_centre, rng):
     """Facet centres (RA, Dec in arcsec) uniform within radius_deg of the phase centre."""
     ra0, dec0 = phase_centre[0] * ARCSEC, phase_centre[1] * ARCSEC
