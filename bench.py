@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--channels-per-stream", type=int, default=0)
     ap.add_argument("--rows-per-stream", type=int, default=0)
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="before timing, copy this many MiB device to device once (a plain copy "
+                         "of known size that scales HBM byte counters)")
     ap.add_argument("--workload", choices=["faceting", "wproj", "adjoint"], default="faceting",
                     help="faceting: the 25-facet Jones workload (default); wproj: the 4096^2 "
                          "w-projection workload; adjoint: chi-squared value and gradient of a "
@@ -51,6 +54,20 @@ def parse():
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB: reads and writes exactly mib * 1024 KiB."""
+    if mib <= 0:
+        return
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy: %d MiB" % mib)
+    del a, b
 
 
 def rel_errors(got, want):
@@ -111,6 +128,7 @@ def main():
     set_opts(PATH_FP32_KAHAN)
     check(lib.initLibrary(b.byref()), "initLibrary")
     stats = bullseye_stats()
+    calibration_copy(args.calibration_copy_mib)
 
     def run(path, steps, warmup):
         set_opts(path)
@@ -252,6 +270,7 @@ def main_wproj(args):
     check(lib.initLibrary(b.byref()), "initLibrary")
     log("initLibrary (tap re-layout) %.1fs" % (time.time() - t))
     stats = bullseye_stats()
+    calibration_copy(args.calibration_copy_mib)
 
     def run(path, steps, warmup):
         set_opts(path)

@@ -117,17 +117,27 @@ template <typename T, bool KAHAN> struct acc1 {
 // Sample s of a stream of nrows rows x ncb channels -> (row offset tr, channel offset j).
 // Channels inner: rows in order, channels serpentine; time inner: channels in order, rows
 // serpentine.  Either way consecutive samples are neighbours in (time, frequency).
+// Written with selects only: swapping the outputs through branches makes the compiler build a
+// dynamically indexed stack array (scratch memory) for them.
 __device__ __forceinline__ void stream_sample(uint32_t s, uint32_t nrows, uint32_t ncb,
                                               bool time_inner, uint32_t &tr, uint32_t &j) {
-    if (time_inner) {
-        j = s / nrows;
-        tr = s - j * nrows;
-        if (j & 1) tr = nrows - 1 - tr;
-    } else {
-        tr = s / ncb;
-        j = s - tr * ncb;
-        if (tr & 1) j = ncb - 1 - j;
-    }
+    const uint32_t inner = time_inner ? nrows : ncb;
+    const uint32_t o = s / inner;
+    const uint32_t i0 = s - o * inner;
+    const uint32_t i = (o & 1) ? inner - 1 - i0 : i0;
+    tr = time_inner ? i : o;
+    j = time_inner ? o : i;
+}
+
+// Workgroups are dispatched to the XCDs round-robin (block b runs on XCD b % NXCD).  This
+// renumbering is a bijection of [0, gridDim.x) under which consecutive logical blocks run on one
+// XCD, so work items that read the same rows (the facets of one baseline segment) share that
+// XCD's L2 instead of each XCD fetching them.  Results do not depend on it.
+constexpr uint32_t NXCD = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    const uint32_t x = b % NXCD, i = b / NXCD, q = n / NXCD, r = n % NXCD;
+    return x * q + (x < r ? x : r) + i;
 }
 
 // ---------- publication of per-lane partial sums ---------------------------------------------

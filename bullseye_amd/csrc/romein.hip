@@ -222,7 +222,7 @@ template <int MODE, int PATH, bool WPROJ> struct Gridder {
         const int gl = lane - gw * G;
         if (gw >= gpw) return;
         const uint64_t stream =
-            ((uint64_t)blockIdx.x * (BLOCK / 64) + wave) * (uint64_t)gpw + (uint64_t)gw;
+            ((uint64_t)xcd_block() * (BLOCK / 64) + wave) * (uint64_t)gpw + (uint64_t)gw;
         if (stream >= a.n_streams) return;
         // stream -> (baseline, channel block, segment, facet); facet fastest so that the
         // facets of one baseline share visibility reads through L2

@@ -306,7 +306,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         const int gl = lane - gw * G;
         if (gw >= GPW) return;
         const uint64_t stream =
-            ((uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
+            ((uint64_t)xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
         if (stream >= a.n_streams) return;
         uint64_t t = stream;
         const uint32_t facet = (uint32_t)(t % a.facets);

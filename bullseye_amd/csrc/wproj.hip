@@ -252,7 +252,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         const int slot = threadIdx.x / L;            // stream slot in the block
         const int sl = threadIdx.x - slot * L;       // lane within the stream
         const bool in_slot = slot < (int)w.streams_per_block;
-        const uint64_t li = (uint64_t)blockIdx.x * w.streams_per_block + slot;
+        const uint64_t li = (uint64_t)xcd_block() * w.streams_per_block + slot;
         const bool live = in_slot && li < w.n_list;
         R_t *recs = reinterpret_cast<R_t *>(smem) + slot * REC;
         __shared__ uint32_t s_steps;
@@ -554,7 +554,7 @@ template <int MODE, int PATH, int M> struct WColGridder {
         const int gl = lane - gw * G;
         if (gw >= GPW) return;
         const uint64_t li =
-            ((uint64_t)blockIdx.x * (WCOL_BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
+            ((uint64_t)xcd_block() * (WCOL_BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
         if (li >= w.n_list) return;
         uint64_t t = w.stream_list[li];
         const uint32_t facet = (uint32_t)(t % a.facets);
