@@ -7,11 +7,13 @@
 //    the v-tap for the accumulator that owns y = j mod W), so the walk is loads + MACs;
 //  * 4-correlation modes split the correlations over lanes (CPL = 1 per lane on the FP32
 //    paths, 2 on the FP64 path), which cuts the accumulator registers per lane;
-//  * FP32 compensated sums take one Kahan step per run of QUAD samples on an unchanged
-//    footprint: the QUAD products are chained by FMA onto -c and the result is added with
-//    the Kahan correction (footprints move every few tens of samples along a baseline, so
-//    nearly every run is a full QUAD).  The error stays O(u) per term, independent of the
-//    number of samples a register sums.
+//  * FP32 compensated sums take one Kahan step per run of samples on an unchanged footprint:
+//    the run's products are chained by FMA onto -c and the result is added with the Kahan
+//    correction.  A run is made of uniform quads (QUAD consecutive samples whose valid members
+//    share one footprint), at most one step of G samples long; the leader lane of each quad
+//    decides uniformity once per step, so the walking lanes read one flag per quad.  Mixed
+//    quads take one Kahan step per sample.  The error stays O(u) per term times the run
+//    length, independent of the number of samples a register sums.
 // A lane group is W * NSPLIT lanes: lane (h, a) owns u-residue a and correlations
 // [h*CPL, (h+1)*CPL) of all W v-residues.  Each group's LDS records start on a different
 // LDS bank so the four groups of a wavefront read without conflicts.
@@ -31,7 +33,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <typename T, int NC, int W> struct Rec {
-    int x0, y0, cgi, pad;
+    int x0, y0, cgi, qf;    // footprint; qf (first record of a quad): 1 = the quad is uniform
+    int qx0, qy0, qcgi, pad;  // the uniform quad's footprint (qcgi < 0: no valid sample)
     cpx<T> vis[NC];
     T cu[W];
     T cv[W];
@@ -245,9 +248,10 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
     }
 
-    // QUAD samples on one footprint into the compensated sums with a single Kahan step.
-    static __device__ __forceinline__ void add_quad(const R_t *recs, int ua, int c0, Acc &acc) {
-        T y[W][CPL][2];
+    typedef T Run[W][CPL][2];
+
+    // A run starts from -c of every register sum and ends with one Kahan step.
+    static __device__ __forceinline__ void open_run(const Acc &acc, Run &y) {
 #pragma unroll
         for (int r = 0; r < W; ++r)
 #pragma unroll
@@ -255,6 +259,20 @@ template <int MODE, int PATH, int W> struct GridderW {
                 y[r][k][0] = -acc[r][k][0].c;
                 y[r][k][1] = -acc[r][k][1].c;
             }
+    }
+
+    static __device__ __forceinline__ void close_run(Acc &acc, const Run &y) {
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                acc[r][k][0].add_pre(y[r][k][0]);
+                acc[r][k][1].add_pre(y[r][k][1]);
+            }
+    }
+
+    // QUAD samples on the run's footprint: products chained by FMA onto the run.
+    static __device__ __forceinline__ void run_quad(const R_t *recs, int ua, int c0, Run &y) {
 #pragma unroll
         for (int q = 0; q < QUAD; ++q) {
             const R_t &rec = recs[q];
@@ -276,13 +294,6 @@ template <int MODE, int PATH, int W> struct GridderW {
                 }
             }
         }
-#pragma unroll
-        for (int r = 0; r < W; ++r)
-#pragma unroll
-            for (int k = 0; k < CPL; ++k) {
-                acc[r][k][0].add_pre(y[r][k][0]);
-                acc[r][k][1].add_pre(y[r][k][1]);
-            }
     }
 
     // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
@@ -383,16 +394,17 @@ template <int MODE, int PATH, int W> struct GridderW {
             const int kmax = (int)min((uint32_t)G, nsamp - step0);
             int kk = 0;
             if constexpr (KAHAN) {
-                while (kk + QUAD <= kmax) {
-                    // a run of QUAD records whose valid members share one footprint; invalid
-                    // records carry zero taps.  The footprint is that of the first valid one.
+                const int nq = kmax / QUAD;
+                // the leader of each full quad decides, once, whether the quad's valid samples
+                // share one footprint
+                if ((gl % QUAD) == 0 && gl / QUAD < nq) {
                     int cg = -1, x0 = 0, y0 = 0;
                     bool same = true;
 #pragma unroll
                     for (int q = 0; q < QUAD; ++q) {
-                        const int c1 = grp[kk + q].cgi;
+                        const int c1 = grp[gl + q].cgi;
                         if (c1 < 0) continue;
-                        const int x1 = grp[kk + q].x0, y1 = grp[kk + q].y0;
+                        const int x1 = grp[gl + q].x0, y1 = grp[gl + q].y0;
                         if (cg < 0) {
                             cg = c1;
                             x0 = x1;
@@ -401,13 +413,44 @@ template <int MODE, int PATH, int W> struct GridderW {
                             same = same && c1 == cg && x1 == x0 && y1 == y0;
                         }
                     }
-                    if (!same) break;
-                    if (cg >= 0) {
-                        move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
-                        add_quad(grp + kk, ua, c0, acc);
-                    }
-                    kk += QUAD;
+                    grp[gl].qf = same ? 1 : 0;
+                    grp[gl].qx0 = x0;
+                    grp[gl].qy0 = y0;
+                    grp[gl].qcgi = cg;
                 }
+                wave_sync();
+                Run y;
+                bool open = false;
+                for (int q = 0; q < nq; ++q) {
+                    const R_t &L = grp[q * QUAD];
+                    if (L.qf) {
+                        if (L.qcgi < 0) continue;   // no valid sample in the quad
+                        if (open && (L.qx0 != px0 || L.qy0 != py0 || L.qcgi != pcgi)) {
+                            close_run(acc, y);
+                            open = false;
+                        }
+                        if (!open) {
+                            move_to(a, facet, ua, c0, L.qx0, L.qy0, L.qcgi, px0, py0, pcgi, acc);
+                            open_run(acc, y);
+                            open = true;
+                        }
+                        run_quad(grp + q * QUAD, ua, c0, y);
+                    } else {
+                        if (open) {
+                            close_run(acc, y);
+                            open = false;
+                        }
+#pragma unroll
+                        for (int k2 = 0; k2 < QUAD; ++k2) {
+                            const R_t &rr = grp[q * QUAD + k2];
+                            if (rr.cgi < 0) continue;
+                            move_to(a, facet, ua, c0, rr.x0, rr.y0, rr.cgi, px0, py0, pcgi, acc);
+                            add_one(rr, ua, c0, acc);
+                        }
+                    }
+                }
+                if (open) close_run(acc, y);
+                kk = nq * QUAD;
             }
             for (; kk < kmax; ++kk) {
                 const int cg = grp[kk].cgi;
