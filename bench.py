@@ -70,6 +70,20 @@ def calibration_copy(mib):
     del a, b
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch of the workload's gridding kernel(s): rocprofv3 FETCH_SIZE plus
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB device copy, as summarised in
+    the committed profile `profiles/<name>`; None when the summary is absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
 def rel_errors(got, want):
     import numpy as np
     out = []
@@ -180,7 +194,8 @@ def main():
         "rms_err_vs_fp64_gpu": rms_vs_gpu64, "peak_err_vs_fp64_gpu": peak_vs_gpu64,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("romein_w_facet_4_cor_corrections_fp32comp_pmc.md"),
                      "kernel_ms": round(k32, 3),
                      "flops_per_launch": flops},
     }
@@ -316,7 +331,8 @@ def main_wproj(args):
         "roofline": {"bound": "mfma", "achieved": round(flops / (k32 * 1e-3) / 1e12, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(flops / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel_ms": round(k32, 3), "flops_per_launch": flops},
+                     "traffic": profiled_traffic("wproj_grid_single_pol_fp32comp_pmc.md"),
+                     "kernel_ms": round(k32, 3), "flops_per_launch": flops},
     }
     if not args.no_cpu:
         from oracle import oracle
