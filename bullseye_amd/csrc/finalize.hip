@@ -186,6 +186,76 @@ hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Stokes formation (bullseye/data/stokes.py:90-172): for every (facet, channel) the value the
+// pipeliner stores in correlation slot 0, (g[a] + sign * g[b]) / 2 in the grid type, or slot 0
+// itself when first < 0.  tmp: [facet][cube][npix] complex.
+__global__ void k_stokes_combine(const grid_base_type *g, grid_base_type *tmp, uint32_t F,
+                                 uint32_t C, uint32_t P, uint64_t npix, int first, int second,
+                                 int sign) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)F * C * npix) return;
+    const uint64_t fc = i / npix, px = i - fc * npix;
+    const grid_base_type *base = g + 2 * (fc * P * npix + px);
+    grid_base_type re, im;
+    if (first < 0) {
+        re = base[0];
+        im = base[1];
+    } else {
+        const grid_base_type *a = base + 2 * (uint64_t)first * npix;
+        const grid_base_type *b = base + 2 * (uint64_t)second * npix;
+        re = sign > 0 ? a[0] + b[0] : a[0] - b[0];
+        im = sign > 0 ? a[1] + b[1] : a[1] - b[1];
+        re /= (grid_base_type)2;
+        im /= (grid_base_type)2;
+    }
+    tmp[2 * i] = re;
+    tmp[2 * i + 1] = im;
+}
+
+// The pipeliner's result after Stokes formation and compaction (bullseye_pipeliner.py:406-416),
+// decided per destination slot t of a facet's [cube][P] block: with compaction, slot (P-1)*c
+// receives channel c's Stokes value (for P = 1 every channel targets slot 0 and the last one
+// stays); otherwise slot 0 of channel c (t = P*c) receives it; every other slot is unchanged.
+// contiguous = 1 places channel c at slot c instead.
+__global__ void k_stokes_place(grid_base_type *g, const grid_base_type *tmp, uint32_t F,
+                               uint32_t C, uint32_t P, uint64_t npix, int compact,
+                               int contiguous) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t slots = (uint64_t)C * P;
+    if (i >= (uint64_t)F * slots * npix) return;
+    const uint64_t fs = i / npix, px = i - fs * npix;
+    const uint64_t f = fs / slots, t = fs - f * slots;
+    int64_t c = -1;
+    if (compact) {
+        if (contiguous) {
+            if (t < C) c = (int64_t)t;
+        } else if (P == 1) {
+            if (t == 0) c = (int64_t)C - 1;
+        } else if (t % (P - 1) == 0 && t / (P - 1) < C) {
+            c = (int64_t)(t / (P - 1));
+        }
+    }
+    if (c < 0 && t % P == 0) c = (int64_t)(t / P);
+    if (c < 0) return;
+    const uint64_t src = 2 * ((f * C + (uint64_t)c) * npix + px);
+    g[2 * i] = tmp[src];
+    g[2 * i + 1] = tmp[src + 1];
+}
+
+hipError_t stokes_device(void *g, void *tmp, uint32_t F, uint32_t C, uint32_t P, uint64_t npix,
+                         int first, int second, int sign, int compact, int contiguous,
+                         hipStream_t st) {
+    const uint64_t n1 = (uint64_t)F * C * npix, n2 = n1 * P;
+    if (!n1) return hipSuccess;
+    k_stokes_combine<<<nblk(n1), 256, 0, st>>>((const grid_base_type *)g, (grid_base_type *)tmp,
+                                               F, C, P, npix, first, second, sign);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_stokes_place<<<nblk(n2), 256, 0, st>>>((grid_base_type *)g, (const grid_base_type *)tmp, F,
+                                             C, P, npix, compact, contiguous);
+    return hipGetLastError();
+}
+
 hipError_t normalize_device(void *out, const double *norm, uint64_t slices, uint64_t npix,
                             hipStream_t st) {
     if (!slices || !npix) return hipSuccess;

@@ -48,6 +48,9 @@ hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P
                        uint32_t pstride, uint64_t npix, hipStream_t st);
 hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st);
 hipError_t fold_slots(double *p, uint64_t stride, uint32_t nslots, hipStream_t st);
+hipError_t stokes_device(void *g, void *tmp, uint32_t F, uint32_t C, uint32_t P, uint64_t npix,
+                         int first, int second, int sign, int compact, int contiguous,
+                         hipStream_t st);
 hipError_t normalize_device(void *out, const double *norm, uint64_t slices, uint64_t npix,
                             hipStream_t st);
 hipError_t weight_uniformly_device(void *out, const void *sf, const uint64_t *cgi, uint32_t F,
@@ -103,7 +106,7 @@ struct State {
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
-    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out;
+    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
     DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
@@ -840,7 +843,7 @@ int releaseLibrary(void) {
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
-                      &g.dg_model, &g.dg_out};
+                      &g.dg_model, &g.dg_out, &g.stokes_tmp};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
@@ -1010,6 +1013,36 @@ int facet_4_cor_corrections(gridding_parameters *p) { return grid_call(p, MODE_4
 int grid_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, false, false); }
 // wrapper.cpp:402-425: faceting uvw transform but no phase shift for the sampling function
 int facet_sampling_function(gridding_parameters *p) { return grid_call(p, MODE_PSF, true, false); }
+
+// Stokes formation and compaction of the normalised grids in output_buffer (the pipeliner's
+// stokes.py:90-172 and bullseye_pipeliner.py:406-416 steps), on the device.  Synchronous.
+int bullseye_form_stokes(gridding_parameters *p, int first, int second, int sign, int compact,
+                         int contiguous) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (!p || !p->output_buffer) return fail("bullseye_form_stokes: output_buffer is NULL");
+    const uint32_t P = (uint32_t)g.P;
+    if (first >= 0 && (first >= (int)P || second < 0 || second >= (int)P))
+        return fail("bullseye_form_stokes: correlation slots %d, %d out of range (P = %u)",
+                    first, second, P);
+    if (first >= 0 && sign != 1 && sign != -1) return fail("bullseye_form_stokes: sign must be +-1");
+    if (barrier_impl()) return 1;
+    const uint64_t npix = (uint64_t)g.nx * g.ny;
+    const size_t bytes = g.F * g.cube * P * npix * 2 * sizeof(grid_base_type);
+    HIP_TRY(g.stokes_tmp.ensure(std::max<size_t>(16, g.F * g.cube * npix * 2 * sizeof(grid_base_type))));
+    void *dg = p->output_buffer;
+    if (!dev_mode()) {
+        HIP_TRY(g.out_tmp.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(g.out_tmp.p, p->output_buffer, bytes, hipMemcpyHostToDevice,
+                               g.stream));
+        dg = g.out_tmp.p;
+    }
+    HIP_TRY(stokes_device(dg, g.stokes_tmp.p, (uint32_t)g.F, (uint32_t)g.cube, P, npix, first,
+                          second, sign, compact, contiguous, g.stream));
+    if (!dev_mode())
+        HIP_TRY(hipMemcpyAsync(p->output_buffer, dg, bytes, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
 
 // The degridding adjoint of a gridding entry point (visibility modes).  Synchronous.
 int bullseye_degrid(gridding_parameters *p, int mode, const void *model, void *vis_out) {

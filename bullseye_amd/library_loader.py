@@ -21,7 +21,7 @@ ALL_SYMBOLS = ENTRY_POINTS + ("get_gridding_walltime", "get_inversion_walltime",
                               "gridding_barrier")
 EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_get_options",
                      "bullseye_reset_grids", "bullseye_get_stats", "bullseye_set_wplane_support",
-                     "bullseye_degrid")
+                     "bullseye_degrid", "bullseye_form_stokes")
 
 
 class bullseye_options(ctypes.Structure):

@@ -256,6 +256,18 @@ enum bullseye_grid_mode {
  * device pointers. */
 int bullseye_degrid(gridding_parameters *params, int mode, const void *model, void *vis_out);
 
+/* Stokes formation and compaction of the (normalised) grids in output_buffer, on the device: the
+ * pipeliner's steps between normalize and finalize (bullseye/data/stokes.py:90-172 and
+ * bullseye/bin/bullseye_pipeliner.py:406-416), which the reference runs on host arrays.  For
+ * every (facet, channel), correlation slot 0 becomes (slot[first] + sign * slot[second]) / 2 in
+ * the grid type (first < 0: unchanged, the wanted term was gridded directly).  With compact = 1,
+ * channel c's slot-0 grid is then copied to flat slot (P-1)*c of its facet's [cube][P] block, the
+ * reference's arithmetic (contiguous for P = 2; for P = 1 the last channel ends in slot 0); with
+ * contiguous = 1 it goes to slot c instead.  Every other slot is left as it was.  Synchronous;
+ * with device_resident, output_buffer is a device pointer and nothing crosses PCIe. */
+int bullseye_form_stokes(gridding_parameters *params, int first, int second, int sign,
+                         int compact, int contiguous);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,37 @@ def degrid(binding, mode, model, precision="double"):
     return out
 
 
+def pipeliner_stokes(gridded_vis, jones_corrections, data_correlations, gridded_correlations,
+                     feeds, wanted):
+    """The pipeliner's steps between normalize and finalize on a NumPy copy of the grids
+    [F, cube, P, ny, nx]: bullseye/data/stokes.py:90-172 (slot 0 of every (facet, channel) becomes
+    (slot a +- slot b) / 2 for the feed type and wanted term; nothing when the feed type is
+    undetermined, as it is with Jones corrections) and the compaction of
+    bullseye/bin/bullseye_pipeliner.py:406-416 (for I, Q, U, V: channel c's slot 0 is copied, in
+    channel order, to [f, nc, nf] with nf, nc = divmod-like split of (P - 1) * c by P)."""
+    g = np.array(gridded_vis, copy=True)
+    codes = {"I": 1, "Q": 2, "U": 3, "V": 4, "RR": 5, "RL": 6, "LR": 7, "LL": 8,
+             "XX": 9, "XY": 10, "YX": 11, "YY": 12}
+    pairs = {("circular", "I"): ("RR", "LL", 1), ("circular", "V"): ("RR", "LL", -1),
+             ("circular", "Q"): ("RL", "LR", 1), ("circular", "U"): ("RL", "LR", -1),
+             ("linear", "I"): ("XX", "YY", 1), ("linear", "Q"): ("XX", "YY", -1),
+             ("linear", "U"): ("XY", "YX", 1), ("linear", "V"): ("XY", "YX", -1)}
+    order = list(data_correlations) if jones_corrections else list(gridded_correlations)
+    if (feeds, wanted) in pairs:
+        a, b, sign = pairs[(feeds, wanted)]
+        ia, ib = order.index(codes[a]), order.index(codes[b])
+        two = g.real.dtype.type(2)
+        g[:, :, 0] = (g[:, :, ia] + g[:, :, ib]) / two if sign > 0 else \
+            (g[:, :, ia] - g[:, :, ib]) / two
+    if wanted in ("I", "Q", "U", "V"):
+        P = len(gridded_correlations)
+        for f in range(g.shape[0]):
+            for c in range(g.shape[1]):
+                shift = (P - 1) * c
+                g[f, shift // P, shift % P] = g[f, c, 0]
+    return g
+
+
 def baseline_starts(ant1, ant2, n_ant):
     """Prefix scan of rows per baseline and the stable per-baseline row order
     (gpu_algorithm/wrapper.cu:358-454)."""
