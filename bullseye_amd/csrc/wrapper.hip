@@ -48,6 +48,12 @@ hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P
                        uint32_t pstride, uint64_t npix, hipStream_t st);
 hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st);
 hipError_t fold_slots(double *p, uint64_t stride, uint32_t nslots, hipStream_t st);
+size_t dft_sample_bytes(uint64_t rows, uint32_t chan);
+hipError_t launch_dft_image(const GridArgs &a, uint64_t rows, void *samples, uint32_t nx,
+                            uint32_t ny, double cell_x, double cell_y, double *image,
+                            hipStream_t st);
+hipError_t launch_dft_predict(const GridArgs &a, uint64_t rows, const double *comp,
+                              uint32_t ncomp, void *vis_out, hipStream_t st);
 hipError_t stokes_device(void *g, void *tmp, uint32_t F, uint32_t C, uint32_t P, uint64_t npix,
                          int first, int second, int sign, int compact, int contiguous,
                          hipStream_t st);
@@ -106,7 +112,8 @@ struct State {
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
-    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp;
+    DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp, dft_smp,
+        dft_comp;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
     DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
@@ -843,7 +850,7 @@ int releaseLibrary(void) {
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
-                      &g.dg_model, &g.dg_out, &g.stokes_tmp};
+                      &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
     for (DevBuf *b : bufs) b->release();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
@@ -1042,6 +1049,62 @@ int bullseye_form_stokes(gridding_parameters *p, int first, int second, int sign
         HIP_TRY(hipMemcpyAsync(p->output_buffer, dg, bytes, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     return 0;
+}
+
+// Direct Fourier image of the single-correlation samples (polarization_index), FP64.
+int bullseye_dft_image(gridding_parameters *p, double *image) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (!p || !image) return fail("bullseye_dft_image: NULL argument");
+    if (validate_call(p, MODE_1)) return 1;
+    if (barrier_impl()) return 1;
+    GridArgs a{};
+    if (setup_inputs(p, MODE_1, false, false, false, true, a)) return 1;
+    const uint64_t rows = p->row_count, npix = (uint64_t)p->nx * p->ny;
+    HIP_TRY(g.dft_smp.ensure(std::max<size_t>(32, dft_sample_bytes(rows, (uint32_t)p->channel_count))));
+    double *dimg = image;
+    if (!dev_mode()) {
+        HIP_TRY(g.dg_out.ensure(npix * sizeof(double)));
+        dimg = g.dg_out.as<double>();
+    }
+    const double cx = (double)p->cell_size_x * (double)ARCSEC_TO_RAD;
+    const double cy = (double)p->cell_size_y * (double)ARCSEC_TO_RAD;
+    HIP_TRY(launch_dft_image(a, rows, g.dft_smp.p, (uint32_t)p->nx, (uint32_t)p->ny, cx, cy, dimg,
+                             g.stream));
+    if (!dev_mode())
+        HIP_TRY(hipMemcpyAsync(image, dimg, npix * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return check_device_errors();
+}
+
+// Visibilities of point components (l, m, flux), FP64, into correlation polarization_index.
+int bullseye_dft_predict(gridding_parameters *p, const double *components, uint32_t n,
+                         void *vis_out) {
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (!p || !vis_out || (n && !components)) return fail("bullseye_dft_predict: NULL argument");
+    if (validate_call(p, MODE_1)) return 1;
+    if (barrier_impl()) return 1;
+    GridArgs a{};
+    if (setup_inputs(p, MODE_1, false, false, false, false, a)) return 1;
+    const uint64_t rows = p->row_count;
+    const size_t vbytes = rows * p->channel_count * p->number_of_polarization_terms * 2 *
+                          sizeof(visibility_base_type);
+    const double *dcomp = components;
+    void *dvis = vis_out;
+    if (!dev_mode()) {
+        HIP_TRY(g.dft_comp.ensure(std::max<size_t>(24, (size_t)n * 3 * sizeof(double))));
+        if (n)
+            HIP_TRY(hipMemcpyAsync(g.dft_comp.p, components, (size_t)n * 3 * sizeof(double),
+                                   hipMemcpyHostToDevice, g.stream));
+        HIP_TRY(g.dg_out.ensure(std::max<size_t>(16, vbytes)));
+        HIP_TRY(hipMemcpyAsync(g.dg_out.p, vis_out, vbytes, hipMemcpyHostToDevice, g.stream));
+        dcomp = g.dft_comp.as<double>();
+        dvis = g.dg_out.p;
+    }
+    HIP_TRY(launch_dft_predict(a, rows, dcomp, n, dvis, g.stream));
+    if (!dev_mode())
+        HIP_TRY(hipMemcpyAsync(vis_out, dvis, vbytes, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return check_device_errors();
 }
 
 // The degridding adjoint of a gridding entry point (visibility modes).  Synchronous.

@@ -21,7 +21,8 @@ ALL_SYMBOLS = ENTRY_POINTS + ("get_gridding_walltime", "get_inversion_walltime",
                               "gridding_barrier")
 EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_get_options",
                      "bullseye_reset_grids", "bullseye_get_stats", "bullseye_set_wplane_support",
-                     "bullseye_degrid", "bullseye_form_stokes")
+                     "bullseye_degrid", "bullseye_form_stokes", "bullseye_dft_image",
+                     "bullseye_dft_predict")
 
 
 class bullseye_options(ctypes.Structure):
@@ -89,4 +90,7 @@ def load_library(architecture, precision, share_torch_runtime=True):
     lib.bullseye_get_stats.argtypes = [ctypes.POINTER(bullseye_stats)]
     lib.bullseye_degrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p]
+    lib.bullseye_dft_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.bullseye_dft_predict.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                         ctypes.c_void_p]
     return lib

@@ -268,6 +268,23 @@ int bullseye_degrid(gridding_parameters *params, int mode, const void *model, vo
 int bullseye_form_stokes(gridding_parameters *params, int first, int second, int sign,
                          int compact, int contiguous);
 
+/* Direct Fourier transforms in FP64 (bullseye ships an unused one, gpu_algorithm/dft.h:51-83;
+ * these follow the conventions the gridder and finalize produce).  Pixel (x, y) of an nx x ny
+ * image at cell_size_x/y is the direction l = (x - nx/2) cell_x, m = -(y - ny/2) cell_y (radians
+ * through the gridder's float arcsecond constant).
+ * bullseye_dft_image: image[y][x] = sum over the unflagged, enabled, in-field samples of
+ *   correlation polarization_index of weight * Re(V exp(2 pi i (u l + v m))) -- the kernel-free
+ *   image that gridding + finalize approximate (a brute-force cross-check).  image: double
+ *   [ny][nx].
+ * bullseye_dft_predict: V = sum_k flux_k exp(+2 pi i (u l_k + v m_k + w (n_k - 1))) for every
+ *   (row, channel), written to correlation polarization_index of vis_out (visibility type,
+ *   [rows][chan][npol]; other correlations untouched).  components: double [n][3] (l, m, flux).
+ * Both need initLibrary and use the struct's uvw, wavelengths, weights, flags and selections;
+ * with device_resident the array arguments are device pointers.  Synchronous. */
+int bullseye_dft_image(gridding_parameters *params, double *image);
+int bullseye_dft_predict(gridding_parameters *params, const double *components, uint32_t n,
+                         void *vis_out);
+
 #ifdef __cplusplus
 }
 #endif
