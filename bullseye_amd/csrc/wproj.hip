@@ -21,28 +21,14 @@
 // classes wider than 16 use 1024-lane workgroups with up to PAIRS residue pairs per lane.
 #include <algorithm>
 
-#include "device_common.h"
+#include "wproj_args.h"
 
 namespace bam {
 
 namespace wp {
 
-constexpr int SEG = 32;        // rows per stream segment
+constexpr int SEG = 32;        // default rows per stream segment
 constexpr int REC = 64;        // samples prepared per stream per step
-
-struct WArgs {
-    GridArgs g;                        // shared fields (vis, uvw, flags, grid, norm, ...)
-    const void *taps;                  // re-laid taps, complex acc_t
-    const uint64_t *plane_off;         // [wplanes] element offset of plane p's blocks
-    const uint32_t *plane_half;        // [wplanes] S_p
-    const double *plane_norm;          // [wplanes][nf][nf] sum of Re(taps) per block
-    const uint32_t *stream_list;       // streams of this class
-    uint32_t n_list;
-    uint32_t wplanes, nf, fbase;       // fraction index range [fbase, fbase + nf)
-    uint32_t M;                        // class width (odd)
-    uint32_t lanes_per_stream, streams_per_block;
-    float wmax_f;
-};
 
 template <typename T, int NC> struct WRec {
     int x0, y0, xm, ym;     // class-footprint origin and its residues mod M
@@ -58,7 +44,7 @@ __device__ __forceinline__ int mod_m(int v, int M) {
 
 // Support class of every stream: the widest plane (2 S_p + 1) any of its samples can use.
 __global__ void k_stream_class(GridArgs a, const uint32_t *plane_half, uint32_t wplanes,
-                               float wmax_f, uint32_t nseg_max, uint8_t *cls) {
+                               float wmax_f, uint32_t nseg_max, uint32_t seg_rows, uint8_t *cls) {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t n = (uint64_t)a.baselines * a.n_chan_blocks * nseg_max;
     if (s >= n) return;
@@ -67,7 +53,7 @@ __global__ void k_stream_class(GridArgs a, const uint32_t *plane_half, uint32_t 
     const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
     const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
     const uint32_t b0 = a.bl_start[bl], nb = a.bl_start[bl + 1] - b0;
-    const uint32_t r_lo = seg * SEG, r_hi = min(nb, r_lo + SEG);
+    const uint32_t r_lo = seg * seg_rows, r_hi = min(nb, r_lo + seg_rows);
     const uint32_t c_lo = cb * a.chan_per_stream, c_hi = min(a.chan, c_lo + a.chan_per_stream);
     int best = 0;
     for (uint32_t r = r_lo; r < r_hi; ++r) {
@@ -245,15 +231,34 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         }
     }
 
+    // Every thread of the block runs this the same number of times (workgroup barriers).
     static __device__ void run(const WArgs &w, unsigned char *smem) {
-        const GridArgs &a = w.g;
+        if (!w.queue) {
+            run_item(w, xcd_block(), smem);
+            return;
+        }
+        __shared__ uint32_t s_item;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_item = atomicAdd(w.queue, 1u);
+            __syncthreads();
+            const uint32_t item = s_item;
+            if (item >= w.n_items) break;
+            run_item(w, item, smem);
+        }
+    }
+
+    static __device__ void run_item(const WArgs &w, uint64_t item, unsigned char *smem) {
         const int M = (int)w.M;
         const int L = (int)w.lanes_per_stream;
         const int slot = threadIdx.x / L;            // stream slot in the block
         const int sl = threadIdx.x - slot * L;       // lane within the stream
         const bool in_slot = slot < (int)w.streams_per_block;
-        const uint64_t li = (uint64_t)xcd_block() * w.streams_per_block + slot;
+        const uint64_t li = item * w.streams_per_block + slot;
         const bool live = in_slot && li < w.n_list;
+        uint32_t stream = 0, fidx = 0;
+        const WArgs &wf = live ? entry_args(w, li, stream, fidx) : w;
+        const GridArgs &a = wf.g;
         R_t *recs = reinterpret_cast<R_t *>(smem) + slot * REC;
         __shared__ uint32_t s_steps;
         if (threadIdx.x == 0) s_steps = 0;
@@ -262,7 +267,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         uint32_t facet = 0, b0 = 0, r_lo = 0, c_lo = 0, ncb = 1, nsamp = 0, nrows = 0;
         const bool tin = *a.time_inner != 0;
         if (live) {
-            uint64_t t = w.stream_list[li];
+            uint64_t t = stream;
             facet = (uint32_t)(t % a.facets);
             t /= a.facets;
             const uint32_t seg = (uint32_t)(t % a.n_segments);
@@ -271,8 +276,8 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
             const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
             b0 = a.bl_start[bl];
             const uint32_t nb = a.bl_start[bl + 1] - b0;
-            r_lo = seg * SEG;
-            const uint32_t r_hi = min(nb, r_lo + SEG);
+            r_lo = seg * wf.seg_rows;
+            const uint32_t r_hi = min(nb, r_lo + wf.seg_rows);
             c_lo = cb * a.chan_per_stream;
             ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
             nrows = r_hi > r_lo ? r_hi - r_lo : 0;
@@ -307,7 +312,9 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
 #pragma unroll
         for (int k = 0; k < NC; ++k) nacc[k] = 0;
         unsigned long long cnt = 0, taps = 0;
-        const cpx<T> *tapbase = reinterpret_cast<const cpx<T> *>(w.taps);
+        const cpx<T> *tapbase = reinterpret_cast<const cpx<T> *>(wf.taps);
+        // normalisation key: facet slice, or the global facet of a batched entry
+        const int kbase = w.facet_args ? (int)fidx : (int)(facet * a.cube);
 
         for (uint32_t st = 0; st < steps; ++st) {
             const uint32_t step0 = st * REC;
@@ -321,12 +328,12 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
                     const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (WPrep<MODE, PATH>::prep(w, M, facet, row, c, rec, nw)) {
+                    if (WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
                         ++cnt;
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
-                                if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
+                                if (ncgi >= 0) flush_norm_lane<NC>(w.g, kbase + ncgi, nacc);
                                 ncgi = rec.cgi;
                             }
 #pragma unroll
@@ -371,10 +378,10 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p)
             if (own[p] && pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
-        // every thread of the block gets here
-        if constexpr (MODE != MODE_PSF)
-            publish_norm<NC>(a, ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1, nacc);
-        publish_counts(a, cnt, taps);
+        // every thread of the block gets here; sums go to the launch's slots (batched facets
+        // share one normalisation array, keyed by global facet)
+        if constexpr (MODE != MODE_PSF) publish_norm<NC>(w.g, ncgi >= 0 ? kbase + ncgi : -1, nacc);
+        publish_counts(w.g, cnt, taps);
     }
 };
 
@@ -533,30 +540,48 @@ template <int MODE, int PATH, int M> struct WColGridder {
             }
     }
 
+    // Publication needs every lane of the wave: walk() returns early for lanes without a
+    // stream.  With a queue each wave takes items (GPW streams) until the list is exhausted.
     static __device__ void run(const WArgs &w, unsigned char *swave) {
-        const GridArgs &a = w.g;
+        if (!w.queue) {
+            item(w, (uint64_t)xcd_block() * (WCOL_BLOCK / 64) + (threadIdx.x >> 6), swave);
+            return;
+        }
+        const int lane = threadIdx.x & 63;
+        for (;;) {
+            uint32_t it = 0;
+            if (lane == 0) it = atomicAdd(w.queue, 1u);
+            it = __shfl(it, 0, 64);
+            if (it >= w.n_items) break;
+            item(w, it, swave);
+        }
+    }
+
+    static __device__ __forceinline__ void item(const WArgs &w, uint64_t it, unsigned char *swave) {
         int nkey = -1;
         double nacc[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nacc[k] = 0;
         unsigned long long cnt = 0, taps = 0;
-        walk(w, swave, nkey, nacc, cnt, taps);
-        if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
-        publish_counts(a, cnt, taps);
+        walk(w, it, swave, nkey, nacc, cnt, taps);
+        if constexpr (MODE != MODE_PSF) publish_norm<NC>(w.g, nkey, nacc);
+        publish_counts(w.g, cnt, taps);
     }
 
-    static __device__ __forceinline__ void walk(const WArgs &w, unsigned char *swave, int &nkey,
-                                                double (&nacc)[NC], unsigned long long &cnt,
+    static __device__ __forceinline__ void walk(const WArgs &w, uint64_t it, unsigned char *swave,
+                                                int &nkey, double (&nacc)[NC],
+                                                unsigned long long &cnt,
                                                 unsigned long long &ntaps) {
-        const GridArgs &a = w.g;
         const int lane = threadIdx.x & 63;
         const int gw = lane / G;
         const int gl = lane - gw * G;
         if (gw >= GPW) return;
-        const uint64_t li =
-            ((uint64_t)xcd_block() * (WCOL_BLOCK / 64) + (threadIdx.x >> 6)) * (uint64_t)GPW + gw;
+        const uint64_t li = it * (uint64_t)GPW + gw;
         if (li >= w.n_list) return;
-        uint64_t t = w.stream_list[li];
+        uint32_t stream, fidx;
+        const WArgs &wf = entry_args(w, li, stream, fidx);
+        const GridArgs &a = wf.g;
+        uint64_t t = stream;
         const uint32_t facet = (uint32_t)(t % a.facets);
         t /= a.facets;
         const uint32_t seg = (uint32_t)(t % a.n_segments);
@@ -565,18 +590,19 @@ template <int MODE, int PATH, int M> struct WColGridder {
         const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
         const uint32_t b0 = a.bl_start[bl];
         const uint32_t nb = a.bl_start[bl + 1] - b0;
-        const uint32_t r_lo = seg * SEG;
-        const uint32_t r_hi = min(nb, r_lo + SEG);
+        const uint32_t r_lo = seg * wf.seg_rows;
+        const uint32_t r_hi = min(nb, r_lo + wf.seg_rows);
         const uint32_t c_lo = cb * a.chan_per_stream;
         const uint32_t ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
         const uint32_t nrows = r_hi > r_lo ? r_hi - r_lo : 0;
         const uint32_t nsamp = nrows * ncb;
         const bool tin = *a.time_inner != 0;
+        const int kbase = w.facet_args ? (int)fidx : (int)(facet * a.cube);
 
         const int ua = gl % M;          // u residue of this lane
         const int c0 = (gl / M) * CPL;  // first correlation of this lane
         R_t *grp = reinterpret_cast<R_t *>(swave + gw * GSTRIDE);
-        const cpx<T> *taps = reinterpret_cast<const cpx<T> *>(w.taps);
+        const cpx<T> *taps = reinterpret_cast<const cpx<T> *>(wf.taps);
 
         Acc acc;
 #pragma unroll
@@ -600,12 +626,12 @@ template <int MODE, int PATH, int M> struct WColGridder {
                     const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (WPrep<MODE, PATH>::prep(w, M, facet, row, c, rec, nw)) {
+                    if (WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
                         ++cnt;
                         ntaps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
-                                if (ncgi >= 0) flush_norm_lane<NC>(a, (int)(facet * a.cube + ncgi), nacc);
+                                if (ncgi >= 0) flush_norm_lane<NC>(w.g, kbase + ncgi, nacc);
                                 ncgi = rec.cgi;
                             }
 #pragma unroll
@@ -668,7 +694,7 @@ template <int MODE, int PATH, int M> struct WColGridder {
 #pragma unroll
             for (int r = 0; r < M; ++r) flush1(a, facet, pcgi, xo, py0 + mod_m(r - pym), c0, acc[r]);
         }
-        nkey = ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1;
+        nkey = ncgi >= 0 ? kbase + ncgi : -1;
     }
 };
 
@@ -679,13 +705,33 @@ __global__ __launch_bounds__(WCOL_BLOCK) void wcol_kernel(WArgs w) {
     Gd::run(w, smem + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
+// Blocks of a persistent launch: as many as fit on the device at once (occupancy), no more
+// than the items need.
+template <typename K>
+static uint64_t persistent_blocks(K kernel, uint32_t block, size_t smem, uint64_t needed) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 1;
+    }
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)block, smem) != hipSuccess ||
+        per <= 0)
+        per = 1;
+    return std::min<uint64_t>(needed, (uint64_t)per * (uint64_t)cus);
+}
+
 template <int MODE, int PATH, int M>
 static hipError_t launch_col(const WArgs &w, hipStream_t st) {
     typedef WColGridder<MODE, PATH, M> Gd;
     const uint64_t waves = (w.n_list + Gd::GPW - 1) / Gd::GPW;
-    const uint64_t blocks = (waves + WCOL_BLOCK / 64 - 1) / (WCOL_BLOCK / 64);
+    uint64_t blocks = (waves + WCOL_BLOCK / 64 - 1) / (WCOL_BLOCK / 64);
     if (!blocks) return hipSuccess;
     const size_t smem = (size_t)(WCOL_BLOCK / 64) * Gd::WAVE_BYTES;
+    if (w.queue)
+        blocks = persistent_blocks(wcol_kernel<MODE, PATH, M>, WCOL_BLOCK, smem, blocks);
     wcol_kernel<MODE, PATH, M><<<dim3((uint32_t)blocks), dim3(WCOL_BLOCK), smem, st>>>(w);
     return hipGetLastError();
 }
@@ -714,9 +760,10 @@ __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
 template <int MODE, int PATH, int PAIRS>
 static hipError_t launch_t(const WArgs &w, uint32_t block, hipStream_t st) {
     typedef typename WGridder<MODE, PATH, PAIRS>::R_t R_t;
-    const uint64_t blocks = (w.n_list + w.streams_per_block - 1) / w.streams_per_block;
+    uint64_t blocks = (w.n_list + w.streams_per_block - 1) / w.streams_per_block;
     if (!blocks) return hipSuccess;
     const size_t smem = (size_t)w.streams_per_block * REC * sizeof(R_t);
+    if (w.queue) blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS>, block, smem, blocks);
     wproj_kernel<MODE, PATH, PAIRS><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
     return hipGetLastError();
 }
@@ -763,12 +810,41 @@ void wproj_shape(int M, uint32_t &block, uint32_t &lanes, uint32_t &spb, int &pa
 }
 
 hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, uint32_t wplanes,
-                               float wmax_f, uint32_t nseg_max, uint8_t *cls, hipStream_t st) {
+                               float wmax_f, uint32_t nseg_max, uint32_t seg_rows, uint8_t *cls,
+                               hipStream_t st) {
     const uint64_t n = (uint64_t)a.baselines * a.n_chan_blocks * nseg_max;
     if (!n) return hipSuccess;
     wp::k_stream_class<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, plane_half, wplanes,
-                                                                     wmax_f, nseg_max, cls);
+                                                                     wmax_f, nseg_max, seg_rows,
+                                                                     cls);
     return hipGetLastError();
+}
+
+uint32_t wproj_streams_per_item(int M, int mode, int path) {
+    if (M <= wp::WCOL_MAX) {
+        const int nc = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
+        const int cpl = nc == 4 ? (path == PATH_FP64 ? 2 : 1) : nc;
+        return (uint32_t)(64 / (M * (nc / cpl)));
+    }
+    uint32_t block, lanes, spb;
+    int pairs;
+    wproj_shape(M, block, lanes, spb, pairs);
+    return spb;
+}
+
+hipError_t launch_wproj_class(const wp::WArgs &w0, int mode, int path, hipStream_t st) {
+    wp::WArgs w = w0;
+    uint32_t block;
+    int pairs;
+    wproj_shape((int)w.M, block, w.lanes_per_stream, w.streams_per_block, pairs);
+    switch (mode) {
+        case MODE_1: return wp::launch_m<MODE_1>(w, path, pairs, block, st);
+        case MODE_2: return wp::launch_m<MODE_2>(w, path, pairs, block, st);
+        case MODE_4: return wp::launch_m<MODE_4>(w, path, pairs, block, st);
+        case MODE_4J: return wp::launch_m<MODE_4J>(w, path, pairs, block, st);
+        case MODE_PSF: return wp::launch_m<MODE_PSF>(w, path, pairs, block, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 size_t wproj_record_bytes(int mode, int path) {
@@ -795,17 +871,11 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
     w.fbase = fbase;
     w.M = M;
     w.wmax_f = wmax_f;
-    uint32_t block;
-    int pairs;
-    wproj_shape((int)M, block, w.lanes_per_stream, w.streams_per_block, pairs);
-    switch (mode) {
-        case MODE_1: return wp::launch_m<MODE_1>(w, path, pairs, block, st);
-        case MODE_2: return wp::launch_m<MODE_2>(w, path, pairs, block, st);
-        case MODE_4: return wp::launch_m<MODE_4>(w, path, pairs, block, st);
-        case MODE_4J: return wp::launch_m<MODE_4J>(w, path, pairs, block, st);
-        case MODE_PSF: return wp::launch_m<MODE_PSF>(w, path, pairs, block, st);
-    }
-    return hipErrorInvalidValue;
+    w.seg_rows = wp::SEG;
+    w.facet_args = nullptr;
+    w.queue = nullptr;
+    w.n_items = 0;
+    return launch_wproj_class(w, mode, path, st);
 }
 
 }  // namespace bam

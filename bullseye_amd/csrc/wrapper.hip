@@ -24,7 +24,8 @@ hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, b
                         int *err, hipStream_t st);
 void wproj_shape(int M, uint32_t &block, uint32_t &lanes, uint32_t &spb, int &pairs);
 hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, uint32_t wplanes,
-                               float wmax_f, uint32_t nseg_max, uint8_t *cls, hipStream_t st);
+                               float wmax_f, uint32_t nseg_max, uint32_t seg_rows, uint8_t *cls,
+                               hipStream_t st);
 hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,
                         const uint32_t *plane_half, const double *plane_norm,
                         const uint32_t *stream_list, uint32_t n_list, uint32_t wplanes,
@@ -546,7 +547,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
         if (!(wmax_f > 0)) return fail("wmax_est must be > 0 for w-projection");
         HIP_TRY(g.wcls.ensure(ncls));
         HIP_TRY(launch_stream_class(a, g.wplane_half.as<uint32_t>(), (uint32_t)g.wplanes, wmax_f,
-                                    nsegw, g.wcls.as<uint8_t>(), g.stream));
+                                    nsegw, SEG, g.wcls.as<uint8_t>(), g.stream));
         g.wcls_host.resize(ncls);
         HIP_TRY(hipMemcpyAsync(g.wcls_host.data(), g.wcls.p, ncls, hipMemcpyDeviceToHost,
                                g.stream));
@@ -661,6 +662,71 @@ double get_inversion_walltime(void) { return g.inv_seconds; }
 
 void gridding_barrier(void) { (void)barrier_impl(); }
 
+// W-projection taps: conv[plane][csz][csz] complex -> contiguous W_p x W_p blocks per
+// (plane, frac_v, frac_u), FP32 and FP64, fraction indexes fbase .. fbase + nf - 1 (nearest
+// rounding gives fractions in [OS/2, 3 OS/2]), plus the Re-sum of every block.
+static int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_t OS,
+                       const std::vector<uint32_t> &halves, DevBuf &taps_f, DevBuf &taps_d,
+                       DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, uint32_t &w_nf,
+                       uint32_t &w_fbase) {
+    const size_t csz = (2 * S + 3) + (2 * S + 2) * (OS - 1);
+    w_fbase = (uint32_t)(OS / 2);
+    w_nf = (uint32_t)(OS + 2);
+    const size_t nf = w_nf;
+    std::vector<uint64_t> off(P + 1, 0);
+    for (size_t q = 0; q < P; ++q) {
+        const size_t wq = 2 * halves[q] + 1;
+        off[q + 1] = off[q] + nf * nf * wq * wq;
+    }
+    if (off[P] > 0xffffffffull) return fail("w-projection tap table exceeds 2^32 taps");
+    const size_t nconv = P * csz * csz;
+    std::vector<convolution_base_type> ch(2 * nconv);
+    if (conv_on_device)
+        HIP_TRY(hipMemcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type),
+                          hipMemcpyDeviceToHost));
+    else
+        memcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type));
+    std::vector<double> td(2 * off[P]);
+    std::vector<float> tf(2 * off[P]);
+    std::vector<double> nrm(P * nf * nf, 0.0);
+    for (size_t q = 0; q < P; ++q) {
+        const size_t sp = halves[q], wq = 2 * sp + 1;
+        for (size_t fv = 0; fv < nf; ++fv)
+            for (size_t fu = 0; fu < nf; ++fu) {
+                double sum = 0;
+                for (size_t tv = 0; tv < wq; ++tv)
+                    for (size_t tu = 0; tu < wq; ++tu) {
+                        const size_t iv = w_fbase + fv + (S - sp + tv) * OS;
+                        const size_t iu = w_fbase + fu + (S - sp + tu) * OS;
+                        double re = 0, im = 0;
+                        if (iv < csz && iu < csz) {
+                            const size_t ci = (q * csz + iv) * csz + iu;
+                            re = ch[2 * ci];
+                            im = ch[2 * ci + 1];
+                        }
+                        const size_t ti = off[q] + ((fv * nf + fu) * wq + tv) * wq + tu;
+                        td[2 * ti] = re;
+                        td[2 * ti + 1] = im;
+                        tf[2 * ti] = (float)re;
+                        tf[2 * ti + 1] = (float)im;
+                        sum += re;
+                    }
+                nrm[(q * nf + fv) * nf + fu] = sum;
+            }
+    }
+    HIP_TRY(taps_d.ensure(td.size() * 8));
+    HIP_TRY(taps_f.ensure(tf.size() * 4));
+    HIP_TRY(plane_off.ensure(P * 8));
+    HIP_TRY(plane_half.ensure(P * 4));
+    HIP_TRY(plane_norm.ensure(nrm.size() * 8));
+    HIP_TRY(hipMemcpy(taps_d.p, td.data(), td.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(taps_f.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_off.p, off.data(), P * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_half.p, halves.data(), P * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_norm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
 int initLibrary(gridding_parameters *p) {
     if (g.initialized) return 0;
     if (!p) return fail("NULL params");
@@ -762,7 +828,7 @@ int initLibrary(gridding_parameters *p) {
         // w-projection taps: conv[plane][csz][csz] complex -> contiguous W_p x W_p blocks per
         // (plane, frac_v, frac_u), fraction indexes fbase .. fbase + nf - 1 (nearest rounding
         // gives fractions in [OS/2, 3 OS/2])
-        const size_t P = g.wplanes, S = g.S, OS = g.OS, csz = g.csz;
+        const size_t P = g.wplanes, S = g.S, OS = g.OS;
         g.wplane_half_host.assign(P, (uint32_t)S);
         if (!g.wplane_half_req.empty()) {
             if (g.wplane_half_req.size() != P)
@@ -777,60 +843,9 @@ int initLibrary(gridding_parameters *p) {
             if (g.wplane_half_host[q] > 31)
                 return fail("w-projection plane %zu: half support %u exceeds 31", q,
                             g.wplane_half_host[q]);
-        g.w_fbase = (uint32_t)(OS / 2);
-        g.w_nf = (uint32_t)(OS + 2);
-        const size_t nf = g.w_nf;
-        std::vector<uint64_t> off(P + 1, 0);
-        for (size_t q = 0; q < P; ++q) {
-            const size_t wq = 2 * g.wplane_half_host[q] + 1;
-            off[q + 1] = off[q] + nf * nf * wq * wq;
-        }
-        if (off[P] > 0xffffffffull) return fail("w-projection tap table exceeds 2^32 taps");
-        const size_t nconv = P * csz * csz;
-        std::vector<convolution_base_type> ch(2 * nconv);
-        if (dev_mode())
-            HIP_TRY(hipMemcpy(ch.data(), p->conv, 2 * nconv * sizeof(convolution_base_type),
-                              hipMemcpyDeviceToHost));
-        else
-            memcpy(ch.data(), p->conv, 2 * nconv * sizeof(convolution_base_type));
-        std::vector<double> td(2 * off[P]);
-        std::vector<float> tf(2 * off[P]);
-        std::vector<double> nrm(P * nf * nf, 0.0);
-        for (size_t q = 0; q < P; ++q) {
-            const size_t sp = g.wplane_half_host[q], wq = 2 * sp + 1;
-            for (size_t fv = 0; fv < nf; ++fv)
-                for (size_t fu = 0; fu < nf; ++fu) {
-                    double sum = 0;
-                    for (size_t tv = 0; tv < wq; ++tv)
-                        for (size_t tu = 0; tu < wq; ++tu) {
-                            const size_t iv = g.w_fbase + fv + (S - sp + tv) * OS;
-                            const size_t iu = g.w_fbase + fu + (S - sp + tu) * OS;
-                            double re = 0, im = 0;
-                            if (iv < csz && iu < csz) {
-                                const size_t ci = (q * csz + iv) * csz + iu;
-                                re = ch[2 * ci];
-                                im = ch[2 * ci + 1];
-                            }
-                            const size_t ti = off[q] + ((fv * nf + fu) * wq + tv) * wq + tu;
-                            td[2 * ti] = re;
-                            td[2 * ti + 1] = im;
-                            tf[2 * ti] = (float)re;
-                            tf[2 * ti + 1] = (float)im;
-                            sum += re;
-                        }
-                    nrm[(q * nf + fv) * nf + fu] = sum;
-                }
-        }
-        HIP_TRY(g.wtaps_d.ensure(td.size() * 8));
-        HIP_TRY(g.wtaps_f.ensure(tf.size() * 4));
-        HIP_TRY(g.wplane_off.ensure(P * 8));
-        HIP_TRY(g.wplane_half.ensure(P * 4));
-        HIP_TRY(g.wplane_norm.ensure(nrm.size() * 8));
-        HIP_TRY(hipMemcpy(g.wtaps_d.p, td.data(), td.size() * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(g.wtaps_f.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(g.wplane_off.p, off.data(), P * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(g.wplane_half.p, g.wplane_half_host.data(), P * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(g.wplane_norm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice));
+        if (build_wtaps(p->conv, dev_mode(), P, S, OS, g.wplane_half_host, g.wtaps_f, g.wtaps_d,
+                        g.wplane_off, g.wplane_half, g.wplane_norm, g.w_nf, g.w_fbase))
+            return 1;
     }
     HIP_TRY(hipStreamSynchronize(g.stream));
     g.grid_seconds = g.inv_seconds = 0;
