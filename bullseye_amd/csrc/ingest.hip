@@ -150,6 +150,13 @@ __global__ __launch_bounds__(WALK_THREADS) void k_walk_order(GridArgs a, int ove
     }
 }
 
+// A kernel, not hipMemsetAsync, so that it can be captured into a graph and replayed.
+hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_fill_u32<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
                              hipStream_t st) {
     k_walk_order<<<1, WALK_THREADS, 0, st>>>(a, override_order, time_inner);

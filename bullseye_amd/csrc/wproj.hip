@@ -184,6 +184,10 @@ template <int MODE, int PATH> struct WPrep {
             return false;
         const int plane = (int)pl;
         const int sp = (int)w.plane_half[plane];
+        if (2 * sp + 1 > M) {  // class lists older than the data: never walk outside the class
+            atomicOr(a.error_flag, 8);
+            return false;
+        }
         const int sm = (M - 1) / 2;
         // class footprint [x0, x0 + M) is centred where the reference's taps are
         r.x0 = dx + (int)a.S - sm;
@@ -265,19 +269,21 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         __syncthreads();
         // stream geometry
         uint32_t facet = 0, b0 = 0, r_lo = 0, c_lo = 0, ncb = 1, nsamp = 0, nrows = 0;
+        uint64_t cls_base = 0;
         const bool tin = *a.time_inner != 0;
         if (live) {
             uint64_t t = stream;
             facet = (uint32_t)(t % a.facets);
             t /= a.facets;
-            const uint32_t seg = (uint32_t)(t % a.n_segments);
-            t /= a.n_segments;
+            const uint32_t seg = (uint32_t)(t % w.nseg);
+            t /= w.nseg;
             const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
             const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
+            cls_base = ((uint64_t)bl * a.n_chan_blocks + cb) * w.nseg_base;
             b0 = a.bl_start[bl];
             const uint32_t nb = a.bl_start[bl + 1] - b0;
-            r_lo = seg * wf.seg_rows;
-            const uint32_t r_hi = min(nb, r_lo + wf.seg_rows);
+            r_lo = seg * w.seg_rows;
+            const uint32_t r_hi = min(nb, r_lo + w.seg_rows);
             c_lo = cb * a.chan_per_stream;
             ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
             nrows = r_hi > r_lo ? r_hi - r_lo : 0;
@@ -314,7 +320,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         unsigned long long cnt = 0, taps = 0;
         const cpx<T> *tapbase = reinterpret_cast<const cpx<T> *>(wf.taps);
         // normalisation key: facet slice, or the global facet of a batched entry
-        const int kbase = w.facet_args ? (int)fidx : (int)(facet * a.cube);
+        const int kbase = w.facet_args ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         for (uint32_t st = 0; st < steps; ++st) {
             const uint32_t step0 = st * REC;
@@ -328,7 +334,9 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
                     const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
+                    const bool mine =
+                        !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
+                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
                         ++cnt;
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -584,20 +592,21 @@ template <int MODE, int PATH, int M> struct WColGridder {
         uint64_t t = stream;
         const uint32_t facet = (uint32_t)(t % a.facets);
         t /= a.facets;
-        const uint32_t seg = (uint32_t)(t % a.n_segments);
-        t /= a.n_segments;
+        const uint32_t seg = (uint32_t)(t % w.nseg);
+        t /= w.nseg;
         const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
         const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
+        const uint64_t cls_base = ((uint64_t)bl * a.n_chan_blocks + cb) * w.nseg_base;
         const uint32_t b0 = a.bl_start[bl];
         const uint32_t nb = a.bl_start[bl + 1] - b0;
-        const uint32_t r_lo = seg * wf.seg_rows;
-        const uint32_t r_hi = min(nb, r_lo + wf.seg_rows);
+        const uint32_t r_lo = seg * w.seg_rows;
+        const uint32_t r_hi = min(nb, r_lo + w.seg_rows);
         const uint32_t c_lo = cb * a.chan_per_stream;
         const uint32_t ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
         const uint32_t nrows = r_hi > r_lo ? r_hi - r_lo : 0;
         const uint32_t nsamp = nrows * ncb;
         const bool tin = *a.time_inner != 0;
-        const int kbase = w.facet_args ? (int)fidx : (int)(facet * a.cube);
+        const int kbase = w.facet_args ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         const int ua = gl % M;          // u residue of this lane
         const int c0 = (gl / M) * CPL;  // first correlation of this lane
@@ -626,7 +635,9 @@ template <int MODE, int PATH, int M> struct WColGridder {
                     const uint32_t c = c_lo + j;
                     const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
+                    const bool mine =
+                        !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
+                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
                         ++cnt;
                         ntaps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -779,8 +790,9 @@ static hipError_t launch_p(const WArgs &w, int pairs, uint32_t block, hipStream_
 }
 
 template <int MODE>
-static hipError_t launch_m(const WArgs &w, int path, int pairs, uint32_t block, hipStream_t st) {
-    if (w.M <= (uint32_t)WCOL_MAX) {
+static hipError_t launch_m(const WArgs &w, int path, bool column, int pairs, uint32_t block,
+                           hipStream_t st) {
+    if (column) {
         switch (path) {
             case PATH_FP64: return launch_col_m<MODE, PATH_FP64>(w, st);
             case PATH_F32K: return launch_col_m<MODE, PATH_F32K>(w, st);
@@ -798,11 +810,15 @@ static hipError_t launch_m(const WArgs &w, int path, int pairs, uint32_t block, 
 
 }  // namespace wp
 
-// Lane shape of a class of width M: (block size, lanes per stream, streams per block, pairs).
-void wproj_shape(int M, uint32_t &block, uint32_t &lanes, uint32_t &spb, int &pairs) {
+// Lane shape of a class of width M under the residue-pair kernel: (block size, lanes per
+// stream, streams per block, pairs per lane).  min_pairs raises the pairs per lane above the
+// smallest that fits (fewer lanes per footprint, more streams per block).
+void wproj_shape(int M, int min_pairs, uint32_t &block, uint32_t &lanes, uint32_t &spb,
+                 int &pairs) {
     const int m2 = M * M;
     block = m2 <= 256 ? 256 : 1024;
     pairs = m2 <= 1024 ? 1 : m2 <= 2048 ? 2 : 4;
+    pairs = std::max(pairs, min_pairs == 2 ? 2 : min_pairs >= 4 ? 4 : 1);
     const int per = (m2 + pairs - 1) / pairs;
     // lanes per stream: whole wavefronts, at least one (REC preparers)
     lanes = (uint32_t)std::max(64, (per + 63) / 64 * 64);
@@ -820,29 +836,52 @@ hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, ui
     return hipGetLastError();
 }
 
-uint32_t wproj_streams_per_item(int M, int mode, int path) {
-    if (M <= wp::WCOL_MAX) {
+bool wproj_shape_valid(int M, int shape) {
+    if (shape == WSHAPE_COLUMN) return M <= wp::WCOL_MAX;
+    if (shape == WSHAPE_DEFAULT) return true;
+    if (shape < WSHAPE_PAIRS1 || shape > WSHAPE_PAIRS4) return false;
+    uint32_t block, lanes, spb;
+    int pairs;
+    const int req = shape == WSHAPE_PAIRS1 ? 1 : shape == WSHAPE_PAIRS2 ? 2 : 4;
+    wproj_shape(M, req, block, lanes, spb, pairs);
+    return pairs == req;   // a width too large for the requested pairs is not a distinct shape
+}
+
+static bool use_column(int M, int shape) {
+    return shape == WSHAPE_COLUMN || (shape == WSHAPE_DEFAULT && M <= wp::WCOL_MAX);
+}
+
+static int shape_pairs(int shape) {
+    return shape == WSHAPE_PAIRS2 ? 2 : shape == WSHAPE_PAIRS4 ? 4 : 1;
+}
+
+uint32_t wproj_streams_per_item(int M, int mode, int path, int shape) {
+    if (use_column(M, shape)) {
         const int nc = (mode == MODE_2) ? 2 : (mode == MODE_4 || mode == MODE_4J) ? 4 : 1;
         const int cpl = nc == 4 ? (path == PATH_FP64 ? 2 : 1) : nc;
         return (uint32_t)(64 / (M * (nc / cpl)));
     }
     uint32_t block, lanes, spb;
     int pairs;
-    wproj_shape(M, block, lanes, spb, pairs);
+    wproj_shape(M, shape_pairs(shape), block, lanes, spb, pairs);
     return spb;
 }
 
-hipError_t launch_wproj_class(const wp::WArgs &w0, int mode, int path, hipStream_t st) {
+hipError_t launch_wproj_class(const wp::WArgs &w0, int mode, int path, int shape,
+                              hipStream_t st) {
     wp::WArgs w = w0;
+    if (!wproj_shape_valid((int)w.M, shape) || w.M > 63 || !(w.M & 1)) return hipErrorInvalidValue;
     uint32_t block;
     int pairs;
-    wproj_shape((int)w.M, block, w.lanes_per_stream, w.streams_per_block, pairs);
+    const bool column = use_column((int)w.M, shape);
+    wproj_shape((int)w.M, shape_pairs(shape), block, w.lanes_per_stream, w.streams_per_block,
+                pairs);
     switch (mode) {
-        case MODE_1: return wp::launch_m<MODE_1>(w, path, pairs, block, st);
-        case MODE_2: return wp::launch_m<MODE_2>(w, path, pairs, block, st);
-        case MODE_4: return wp::launch_m<MODE_4>(w, path, pairs, block, st);
-        case MODE_4J: return wp::launch_m<MODE_4J>(w, path, pairs, block, st);
-        case MODE_PSF: return wp::launch_m<MODE_PSF>(w, path, pairs, block, st);
+        case MODE_1: return wp::launch_m<MODE_1>(w, path, column, pairs, block, st);
+        case MODE_2: return wp::launch_m<MODE_2>(w, path, column, pairs, block, st);
+        case MODE_4: return wp::launch_m<MODE_4>(w, path, column, pairs, block, st);
+        case MODE_4J: return wp::launch_m<MODE_4J>(w, path, column, pairs, block, st);
+        case MODE_PSF: return wp::launch_m<MODE_PSF>(w, path, column, pairs, block, st);
     }
     return hipErrorInvalidValue;
 }
@@ -872,10 +911,15 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
     w.M = M;
     w.wmax_f = wmax_f;
     w.seg_rows = wp::SEG;
+    w.nseg = a.n_segments;
+    w.base_rows = wp::SEG;
+    w.nseg_base = a.n_segments;
+    w.filter = 0;
+    w.seg_cls = nullptr;
     w.facet_args = nullptr;
     w.queue = nullptr;
     w.n_items = 0;
-    return launch_wproj_class(w, mode, path, st);
+    return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
 }
 
 }  // namespace bam

@@ -14,81 +14,11 @@
 #include <string>
 #include <vector>
 
-#include "device_common.h"
+#include "host_common.h"
 
 namespace bam {
-hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
-hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st);
-int romein_w_groups_per_wave(int W, int mode, int path);
-hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, bool fp64,
-                        int *err, hipStream_t st);
-void wproj_shape(int M, uint32_t &block, uint32_t &lanes, uint32_t &spb, int &pairs);
-hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, uint32_t wplanes,
-                               float wmax_f, uint32_t nseg_max, uint32_t seg_rows, uint8_t *cls,
-                               hipStream_t st);
-hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,
-                        const uint32_t *plane_half, const double *plane_norm,
-                        const uint32_t *stream_list, uint32_t n_list, uint32_t wplanes,
-                        uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
-                        hipStream_t st);
-int romein_max_width();
-hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
-                             hipStream_t st);
-hipError_t launch_degrid(const GridArgs &a, const void *model, void *out, uint64_t rows,
-                         uint32_t P, const void *taps, const uint64_t *plane_off,
-                         const uint32_t *plane_half, uint32_t wplanes, uint32_t nf,
-                         uint32_t fbase, float wmax_f, int mode, int path, hipStream_t st);
-size_t ingest_workspace_bytes(uint32_t rows, uint32_t n_bl);
-hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, uint32_t n_ant,
-                       uint32_t n_bl, uint32_t *keys, uint32_t *vals, uint32_t *keys_out,
-                       uint32_t *counts, uint32_t *perm, uint32_t *bl_start, void *ws,
-                       size_t ws_bytes, int *err, hipStream_t st);
-hipError_t grid_to_abi(const double *g, void *out, uint64_t slices_fc, uint32_t P,
-                       uint32_t pstride, uint64_t npix, hipStream_t st);
-hipError_t abi_to_grid(const void *in, double *g, uint64_t slices_fc, uint32_t P,
-                       uint32_t pstride, uint64_t npix, hipStream_t st);
-hipError_t fill_f64(double *p, uint64_t n, double v, hipStream_t st);
-hipError_t fold_slots(double *p, uint64_t stride, uint32_t nslots, hipStream_t st);
-size_t dft_sample_bytes(uint64_t rows, uint32_t chan);
-hipError_t launch_dft_image(const GridArgs &a, uint64_t rows, void *samples, uint32_t nx,
-                            uint32_t ny, double cell_x, double cell_y, double *image,
-                            hipStream_t st);
-hipError_t launch_dft_predict(const GridArgs &a, uint64_t rows, const double *comp,
-                              uint32_t ncomp, void *vis_out, hipStream_t st);
-hipError_t stokes_device(void *g, void *tmp, uint32_t F, uint32_t C, uint32_t P, uint64_t npix,
-                         int first, int second, int sign, int compact, int contiguous,
-                         hipStream_t st);
-hipError_t normalize_device(void *out, const double *norm, uint64_t slices, uint64_t npix,
-                            hipStream_t st);
-hipError_t weight_uniformly_device(void *out, const void *sf, const uint64_t *cgi, uint32_t F,
-                                   uint32_t C, uint32_t P, uint32_t sfC, uint64_t npix,
-                                   hipStream_t st);
-hipError_t finalize_device(void *buf, uint32_t F, uint32_t n, uint32_t fstride, uint32_t nx,
-                           uint32_t ny, const double *tx, const double *ty, void *scratch,
-                           float *rtmp, bool place_real, hipStream_t st);
 
-// cpu_gpu_common/cu_common.h:63 -- a float constant in every precision build
-static const float ARCSEC_TO_RAD = (float)(M_PI / (180.0 * 3600.0));
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n ? n : 16);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
-};
 
 struct TimedCall {
     hipEvent_t start, k0, k1, stop;
@@ -133,7 +63,7 @@ struct State {
 };
 static State g;
 
-static int fail(const char *fmt, ...) {
+int fail(const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -143,17 +73,12 @@ static int fail(const char *fmt, ...) {
     return 1;
 }
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_));    \
-    } while (0)
 
 static bool dev_mode() { return g.opts.device_resident != 0; }
 
 // Copy `bytes` from the caller's array to a device staging buffer (host mode) or use it as
 // is (device-resident mode).
-static hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
+hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
     if (!src) {
         *out = nullptr;
         return hipSuccess;
@@ -192,7 +117,7 @@ static size_t psf_elems() { return g.F * g.sfC * g.ny * g.nx * 2; }
 static size_t abi_grid_bytes() { return g.F * g.cube * g.P * g.ny * g.nx * 2 * sizeof(grid_base_type); }
 static size_t abi_psf_bytes() { return g.F * g.sfC * g.ny * g.nx * 2 * sizeof(grid_base_type); }
 
-static int check_device_errors() {
+int check_device_errors() {
     int flag = 0;
     if (g.errflag.p) {
         hipError_t e = hipMemcpy(&flag, g.errflag.p, sizeof(int), hipMemcpyDeviceToHost);
@@ -205,6 +130,9 @@ static int check_device_errors() {
     if (flag & 1) return fail("JONES MATRIX SET CONTAINS SINGULAR MATRICIES");
     if (flag & 2) return fail("antenna id out of range (antenna_count or baseline_count too small)");
     if (flag & 4) return fail("channel grid index, spw id or timestamp id out of range");
+    if (flag & 8)
+        return fail("a w-projection sample is wider than its stream's support class (the data "
+                    "changed after the batch was planned)");
     return 0;
 }
 
@@ -598,6 +526,87 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     return 0;
 }
 
+// W-projection taps: conv[plane][csz][csz] complex -> contiguous W_p x W_p blocks per
+// (plane, frac_v, frac_u), FP32 and FP64, fraction indexes fbase .. fbase + nf - 1 (nearest
+// rounding gives fractions in [OS/2, 3 OS/2]), plus the Re-sum of every block.
+int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_t OS,
+                const std::vector<uint32_t> &halves, DevBuf &taps_f, DevBuf &taps_d,
+                DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, uint32_t &w_nf,
+                uint32_t &w_fbase) {
+    const size_t csz = (2 * S + 3) + (2 * S + 2) * (OS - 1);
+    w_fbase = (uint32_t)(OS / 2);
+    w_nf = (uint32_t)(OS + 2);
+    const size_t nf = w_nf;
+    std::vector<uint64_t> off(P + 1, 0);
+    for (size_t q = 0; q < P; ++q) {
+        const size_t wq = 2 * halves[q] + 1;
+        off[q + 1] = off[q] + nf * nf * wq * wq;
+    }
+    if (off[P] > 0xffffffffull) return fail("w-projection tap table exceeds 2^32 taps");
+    const size_t nconv = P * csz * csz;
+    std::vector<convolution_base_type> ch(2 * nconv);
+    if (conv_on_device)
+        HIP_TRY(hipMemcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type),
+                          hipMemcpyDeviceToHost));
+    else
+        memcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type));
+    std::vector<double> td(2 * off[P]);
+    std::vector<float> tf(2 * off[P]);
+    std::vector<double> nrm(P * nf * nf, 0.0);
+    for (size_t q = 0; q < P; ++q) {
+        const size_t sp = halves[q], wq = 2 * sp + 1;
+        for (size_t fv = 0; fv < nf; ++fv)
+            for (size_t fu = 0; fu < nf; ++fu) {
+                double sum = 0;
+                for (size_t tv = 0; tv < wq; ++tv)
+                    for (size_t tu = 0; tu < wq; ++tu) {
+                        const size_t iv = w_fbase + fv + (S - sp + tv) * OS;
+                        const size_t iu = w_fbase + fu + (S - sp + tu) * OS;
+                        double re = 0, im = 0;
+                        if (iv < csz && iu < csz) {
+                            const size_t ci = (q * csz + iv) * csz + iu;
+                            re = ch[2 * ci];
+                            im = ch[2 * ci + 1];
+                        }
+                        const size_t ti = off[q] + ((fv * nf + fu) * wq + tv) * wq + tu;
+                        td[2 * ti] = re;
+                        td[2 * ti + 1] = im;
+                        tf[2 * ti] = (float)re;
+                        tf[2 * ti + 1] = (float)im;
+                        sum += re;
+                    }
+                nrm[(q * nf + fv) * nf + fu] = sum;
+            }
+    }
+    HIP_TRY(taps_d.ensure(td.size() * 8));
+    HIP_TRY(taps_f.ensure(tf.size() * 4));
+    HIP_TRY(plane_off.ensure(P * 8));
+    HIP_TRY(plane_half.ensure(P * 4));
+    HIP_TRY(plane_norm.ensure(nrm.size() * 8));
+    HIP_TRY(hipMemcpy(taps_d.p, td.data(), td.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(taps_f.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_off.p, off.data(), P * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_half.p, halves.data(), P * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(plane_norm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+LibView lib_view() {
+    LibView v;
+    v.initialized = g.initialized;
+    v.stream = g.stream;
+    v.device_resident = g.opts.device_resident != 0;
+    v.path = g.path;
+    v.walk_order = (int)g.opts.walk_order;
+    v.errflag = g.errflag.as<int>();
+    v.counters = g.counters.as<unsigned long long>();
+    v.nslots = g.nslots;
+    v.last_counters = g.last_counters;
+    v.stats = &g.stats;
+    v.grid_seconds = &g.grid_seconds;
+    return v;
+}
+
 }  // namespace bam
 
 using namespace bam;
@@ -661,71 +670,6 @@ double get_gridding_walltime(void) { return g.grid_seconds; }
 double get_inversion_walltime(void) { return g.inv_seconds; }
 
 void gridding_barrier(void) { (void)barrier_impl(); }
-
-// W-projection taps: conv[plane][csz][csz] complex -> contiguous W_p x W_p blocks per
-// (plane, frac_v, frac_u), FP32 and FP64, fraction indexes fbase .. fbase + nf - 1 (nearest
-// rounding gives fractions in [OS/2, 3 OS/2]), plus the Re-sum of every block.
-static int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_t OS,
-                       const std::vector<uint32_t> &halves, DevBuf &taps_f, DevBuf &taps_d,
-                       DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, uint32_t &w_nf,
-                       uint32_t &w_fbase) {
-    const size_t csz = (2 * S + 3) + (2 * S + 2) * (OS - 1);
-    w_fbase = (uint32_t)(OS / 2);
-    w_nf = (uint32_t)(OS + 2);
-    const size_t nf = w_nf;
-    std::vector<uint64_t> off(P + 1, 0);
-    for (size_t q = 0; q < P; ++q) {
-        const size_t wq = 2 * halves[q] + 1;
-        off[q + 1] = off[q] + nf * nf * wq * wq;
-    }
-    if (off[P] > 0xffffffffull) return fail("w-projection tap table exceeds 2^32 taps");
-    const size_t nconv = P * csz * csz;
-    std::vector<convolution_base_type> ch(2 * nconv);
-    if (conv_on_device)
-        HIP_TRY(hipMemcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type),
-                          hipMemcpyDeviceToHost));
-    else
-        memcpy(ch.data(), conv, 2 * nconv * sizeof(convolution_base_type));
-    std::vector<double> td(2 * off[P]);
-    std::vector<float> tf(2 * off[P]);
-    std::vector<double> nrm(P * nf * nf, 0.0);
-    for (size_t q = 0; q < P; ++q) {
-        const size_t sp = halves[q], wq = 2 * sp + 1;
-        for (size_t fv = 0; fv < nf; ++fv)
-            for (size_t fu = 0; fu < nf; ++fu) {
-                double sum = 0;
-                for (size_t tv = 0; tv < wq; ++tv)
-                    for (size_t tu = 0; tu < wq; ++tu) {
-                        const size_t iv = w_fbase + fv + (S - sp + tv) * OS;
-                        const size_t iu = w_fbase + fu + (S - sp + tu) * OS;
-                        double re = 0, im = 0;
-                        if (iv < csz && iu < csz) {
-                            const size_t ci = (q * csz + iv) * csz + iu;
-                            re = ch[2 * ci];
-                            im = ch[2 * ci + 1];
-                        }
-                        const size_t ti = off[q] + ((fv * nf + fu) * wq + tv) * wq + tu;
-                        td[2 * ti] = re;
-                        td[2 * ti + 1] = im;
-                        tf[2 * ti] = (float)re;
-                        tf[2 * ti + 1] = (float)im;
-                        sum += re;
-                    }
-                nrm[(q * nf + fv) * nf + fu] = sum;
-            }
-    }
-    HIP_TRY(taps_d.ensure(td.size() * 8));
-    HIP_TRY(taps_f.ensure(tf.size() * 4));
-    HIP_TRY(plane_off.ensure(P * 8));
-    HIP_TRY(plane_half.ensure(P * 4));
-    HIP_TRY(plane_norm.ensure(nrm.size() * 8));
-    HIP_TRY(hipMemcpy(taps_d.p, td.data(), td.size() * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(taps_f.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(plane_off.p, off.data(), P * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(plane_half.p, halves.data(), P * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(plane_norm.p, nrm.data(), nrm.size() * 8, hipMemcpyHostToDevice));
-    return 0;
-}
 
 int initLibrary(gridding_parameters *p) {
     if (g.initialized) return 0;
@@ -867,6 +811,7 @@ int releaseLibrary(void) {
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
                       &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
     for (DevBuf *b : bufs) b->release();
+    batch_release_all();
     (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
     g.initialized = false;

@@ -22,7 +22,10 @@ ALL_SYMBOLS = ENTRY_POINTS + ("get_gridding_walltime", "get_inversion_walltime",
 EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_get_options",
                      "bullseye_reset_grids", "bullseye_get_stats", "bullseye_set_wplane_support",
                      "bullseye_degrid", "bullseye_form_stokes", "bullseye_dft_image",
-                     "bullseye_dft_predict")
+                     "bullseye_dft_predict", "bullseye_batch_kernel_set",
+                     "bullseye_batch_set_tuning", "bullseye_batch_get_tuning",
+                     "bullseye_batch_plan", "bullseye_batch_execute", "bullseye_batch_autotune",
+                     "bullseye_batch_release")
 
 
 class bullseye_options(ctypes.Structure):
@@ -50,7 +53,37 @@ class bullseye_stats(ctypes.Structure):
     ]
 
 
+class bullseye_batch_facet(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint32),
+        ("ny", ctypes.c_uint32),
+        ("kernel_set", ctypes.c_uint32),
+        ("sampling_function", ctypes.c_uint32),
+        ("cell_x", ctypes.c_double),
+        ("cell_y", ctypes.c_double),
+        ("ra", ctypes.c_double),
+        ("dec", ctypes.c_double),
+        ("output", ctypes.c_void_p),
+        ("normalization", ctypes.c_void_p),
+    ]
+
+
+class bullseye_batch_tuning(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("channels_per_stream", ctypes.c_uint32),
+        ("base_rows", ctypes.c_uint32),
+        ("tuned", ctypes.c_uint32),
+        ("shape", ctypes.c_uint32 * 32),
+        ("chunk", ctypes.c_uint32 * 32),
+        ("tuned_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+    ]
+
+
 PATH_DEFAULT, PATH_FP64, PATH_FP32_KAHAN, PATH_FP32 = 0, 1, 2, 3
+BATCH_PERSISTENT_GRAPH, BATCH_PERSISTENT, BATCH_PER_FACET = 0, 1, 2
+SHAPE_DEFAULT, SHAPE_COLUMN, SHAPE_PAIRS1, SHAPE_PAIRS2, SHAPE_PAIRS4 = 0, 1, 2, 3, 4
 
 
 def library_path(precision):
@@ -93,4 +126,15 @@ def load_library(architecture, precision, share_torch_runtime=True):
     lib.bullseye_dft_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.bullseye_dft_predict.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                          ctypes.c_void_p]
+    lib.bullseye_batch_kernel_set.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                              ctypes.c_float]
+    lib.bullseye_batch_set_tuning.argtypes = [ctypes.POINTER(bullseye_batch_tuning)]
+    lib.bullseye_batch_get_tuning.argtypes = [ctypes.POINTER(bullseye_batch_tuning)]
+    lib.bullseye_batch_plan.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(bullseye_batch_facet), ctypes.c_uint32,
+                                        ctypes.c_int]
+    lib.bullseye_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.POINTER(bullseye_batch_facet), ctypes.c_uint32,
+                                            ctypes.POINTER(bullseye_batch_tuning)]
     return lib

@@ -285,6 +285,89 @@ int bullseye_dft_image(gridding_parameters *params, double *image);
 int bullseye_dft_predict(gridding_parameters *params, const double *components, uint32_t n,
                          void *vis_out);
 
+/* ---- batched facets (w-projection) ----------------------------------------------------------
+ * The reference grids one facet geometry per initLibrary: every facet of a call shares nx, ny,
+ * cell size and kernels (gridding_parameters.h:45-72), and the pipeliner loops over calls
+ * (bullseye/bin/bullseye_pipeliner.py:350-404).  These entry points grid up to 255 facets of
+ * different size, cell, w-kernel stack and kind (visibilities or sampling function) from one
+ * visibility set in a single plan: the facets' streams are merged into one work queue per
+ * support class and drained by persistent kernels, captured once in a HIP graph and replayed.
+ * Each facet follows the semantics of facet_single_pol / facet_duel_pol / facet_4_cor (or
+ * facet_sampling_function) with w-projection, as if initLibrary had been called with its own
+ * nx, ny, cell size and kernels.  initLibrary must have been called (any geometry); its stream,
+ * options (device_resident, compute_path, walk_order) and error reporting are used. */
+
+/* A w-kernel stack: conv is [wplanes][csz][csz] complex convolution_base_type with
+ * csz = (2S+3) + (2S+2)(OS-1), as initLibrary's conv (device memory with device_resident);
+ * half_support[wplanes] gives each plane's half support (<= S <= 31), NULL: S for all; a
+ * sample's plane is lrint(|w| / wmax * (wplanes - 1)) as in the reference.  id < 64; defining
+ * an id again replaces it (and invalidates the plan). */
+int bullseye_batch_kernel_set(uint32_t id, const void *conv, uint32_t wplanes,
+                              uint32_t conv_support, uint32_t conv_oversample,
+                              const uint32_t *half_support, float wmax);
+
+typedef struct bullseye_batch_facet {
+    uint32_t nx, ny;            /* facet grid size */
+    uint32_t kernel_set;        /* id given to bullseye_batch_kernel_set */
+    uint32_t sampling_function; /* 1: grid the facet's sampling function (PSF) */
+    double cell_x, cell_y;      /* cell size, arcsec */
+    double ra, dec;             /* facet centre, arcsec (as facet_centres) */
+    void *output;               /* complex [cube][ncorr][ny][nx] grid_base_type, not normalised
+                                   (cube = cube_channel_dim_size, or
+                                   sampling_function_channel_count for a PSF facet; ncorr = 1, 2
+                                   or 4 by mode, 1 for PSF); device memory with device_resident;
+                                   NULL: not copied out */
+    double *normalization;      /* host [cube][ncorr] normalisation sums, or NULL */
+} bullseye_batch_facet;
+
+enum bullseye_batch_schedule {
+    BULLSEYE_BATCH_PERSISTENT_GRAPH = 0, /* one persistent launch per (kind, support class) over
+                                            all facets, captured in a HIP graph and replayed */
+    BULLSEYE_BATCH_PERSISTENT = 1,       /* the same launches issued on the stream each time */
+    BULLSEYE_BATCH_PER_FACET = 2         /* one launch per (facet, support class), issued on the
+                                            stream each time: the unbatched baseline */
+};
+
+/* Romein tile choices per support class M = 2i+1 (i < 32): shape[i] is the lane shape
+ * (0 default, 1 column ownership (M <= 15), 2/3/4 residue pairs with at least 1/2/4 pairs per
+ * lane) and chunk[i] the classification segments per work item (1, 2 or 4: the visibility
+ * chunk length is chunk * base_rows rows).  channels_per_stream and base_rows are shared by all
+ * classes (0: automatic, 32). */
+typedef struct bullseye_batch_tuning {
+    uint32_t struct_size;       /* sizeof(bullseye_batch_tuning) */
+    uint32_t channels_per_stream;
+    uint32_t base_rows;
+    uint32_t tuned;             /* 1 when filled by bullseye_batch_autotune */
+    uint32_t shape[32];
+    uint32_t chunk[32];
+    double tuned_ms;            /* autotune: summed device time of the chosen launches */
+    double default_ms;          /* autotune: the same with the default tiles */
+} bullseye_batch_tuning;
+int bullseye_batch_set_tuning(const bullseye_batch_tuning *tuning);
+int bullseye_batch_get_tuning(bullseye_batch_tuning *tuning);
+
+/* Plan a batch: stage the visibility set of params (rows, uvw, vis, weights, flags, channels,
+ * spw, antennas, selections) -- only the struct's visibility fields are read; its nx, ny, cell
+ * sizes, facet list and conv are ignored -- sort rows by baseline, classify every stream per
+ * kernel set, build the work lists and (schedule 0) capture the graph.  mode: one of
+ * BULLSEYE_FACET_SINGLE_POL, BULLSEYE_FACET_DUEL_POL, BULLSEYE_FACET_4_COR for the
+ * visibility facets.  The facet table and its output pointers are kept until the next plan.
+ * With device_resident the caller's arrays are used in place and must stay allocated; without,
+ * they are copied here and later changes are not seen. */
+int bullseye_batch_plan(gridding_parameters *params, int mode, const bullseye_batch_facet *facets,
+                        uint32_t n, int schedule);
+/* Grid the planned batch from zero, copy each facet's grid and normalisation sums out, and fill
+ * bullseye_stats (kernel_ms = device time of the gridding launches).  Synchronous. */
+int bullseye_batch_execute(void);
+/* Time every support class of the batch under each lane shape and chunk length, and each
+ * channels-per-stream / base-rows pair, keep the fastest (as bullseye_batch_set_tuning) and
+ * return it.  Leaves no plan: call bullseye_batch_plan afterwards. */
+int bullseye_batch_autotune(gridding_parameters *params, int mode,
+                            const bullseye_batch_facet *facets, uint32_t n,
+                            bullseye_batch_tuning *result);
+/* Free the plan and kernel sets (releaseLibrary does too). */
+int bullseye_batch_release(void);
+
 #ifdef __cplusplus
 }
 #endif

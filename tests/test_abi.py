@@ -9,7 +9,8 @@ import pytest
 
 from bullseye_amd import INCLUDE_DIR, PRECISIONS
 from bullseye_amd.gridding_parameters import gridding_parameters_class
-from bullseye_amd.library_loader import (bullseye_options, bullseye_stats, library_path,
+from bullseye_amd.library_loader import (bullseye_batch_facet, bullseye_batch_tuning,
+                                         bullseye_options, bullseye_stats, library_path,
                                          load_library)
 from oracle import oracle
 
@@ -70,7 +71,9 @@ def test_options_and_stats_sizes():
 
 def test_options_and_stats_layout_match_header(tmp_path):
     """The ctypes mirrors of the extension structs against the C header, compiled with gcc."""
-    fields = {"bullseye_options": bullseye_options, "bullseye_stats": bullseye_stats}
+    fields = {"bullseye_options": bullseye_options, "bullseye_stats": bullseye_stats,
+              "bullseye_batch_facet": bullseye_batch_facet,
+              "bullseye_batch_tuning": bullseye_batch_tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bullseye_amd.h"',
              'int main(void) {']
     for name, cls in fields.items():
@@ -108,3 +111,11 @@ def test_calls_without_init_fail_cleanly(precision):
     assert b"initLibrary" in lib.bullseye_last_error()
     o = bullseye_options(struct_size=3)
     assert lib.bullseye_set_options(ctypes.byref(o)) != 0
+    facets = (bullseye_batch_facet * 1)()
+    assert lib.bullseye_batch_plan(ctypes.byref(p), 1, facets, 1, 0) != 0
+    assert b"initLibrary" in lib.bullseye_last_error()
+    assert lib.bullseye_batch_execute() != 0
+    assert lib.bullseye_batch_kernel_set(0, None, 2, 3, 4, None, ctypes.c_float(1.0)) != 0
+    t = bullseye_batch_tuning(struct_size=ctypes.sizeof(bullseye_batch_tuning))
+    t.chunk[0] = 3
+    assert lib.bullseye_batch_set_tuning(ctypes.byref(t)) != 0
