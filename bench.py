@@ -45,10 +45,23 @@ def parse():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before timing, copy this many MiB device to device once (a plain copy "
                          "of known size that scales HBM byte counters)")
-    ap.add_argument("--workload", choices=["faceting", "wproj", "adjoint"], default="faceting",
+    ap.add_argument("--workload", choices=["faceting", "wproj", "adjoint", "batch"],
+                    default="faceting",
                     help="faceting: the 25-facet Jones workload (default); wproj: the 4096^2 "
                          "w-projection workload; adjoint: chi-squared value and gradient of a "
-                         "2048^2 facet model through gridding and its degridding adjoint")
+                         "2048^2 facet model through gridding and its degridding adjoint; "
+                         "batch: the w-projection visibilities into 200 uneven facets in one "
+                         "persistent graph launch")
+    ap.add_argument("--batch-facets", type=int, default=200)
+    ap.add_argument("--batch-timesteps", type=int, default=480,
+                    help="timesteps of the w-projection visibilities the batch grids")
+    ap.add_argument("--tune-timesteps", type=int, default=30,
+                    help="timesteps of the sample the batch autotuner times (0: no autotune)")
+    ap.add_argument("--tuning", default="",
+                    help="batch tiles as the JSON 'tuning' object a batch run prints (replaces "
+                         "the autotuner)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch: skip the FP64, default-tile and per-facet passes")
     return ap.parse_args()
 
 
@@ -105,6 +118,8 @@ def main():
         return main_wproj(args)
     if args.workload == "adjoint":
         return main_adjoint(args)
+    if args.workload == "batch":
+        return main_batch(args)
     import numpy as np
     import torch
     from bullseye_amd import kernels, synth
@@ -471,6 +486,218 @@ def main_adjoint(args):
         "grad_rms_err_vs_fp64_gpu": rms, "grad_peak_err_vs_fp64_gpu": pk,
         "loss_fp32": loss32, "loss_fp64": loss64,
     }
+    print(json.dumps(result), flush=True)
+
+def main_batch(args):
+    """Batched uneven facets: the w-projection workload's visibilities (128 antennas, 10 km
+    log-radial, 32 channels 1.0-1.5 GHz, 1 correlation; --batch-timesteps of its 480 timesteps)
+    gridded into --batch-facets facets with sizes log-uniform over 128^2-2048^2 at 1.5", each
+    with one of seven w-kernel stacks (16, 24, 32, 48, 64, 96 or 128 planes, supports 7->63 by
+    |w|, OS x8, complex64) and a quarter of them sampling-function facets.  A step is one
+    bullseye_batch_execute: replay of the plan's HIP graph (clear, one persistent launch per
+    (kind, support class) over all facets) and the copy of every facet's grid to its output."""
+    import numpy as np
+    import torch
+    from bullseye_amd import kernels, synth
+    from bullseye_amd.batch import BatchFacet, FacetBatch
+    from bullseye_amd.binding import Binding, ImagingSpec
+    from bullseye_amd.library_loader import (BATCH_PER_FACET, BATCH_PERSISTENT_GRAPH,
+                                             PATH_FP32_KAHAN, PATH_FP64, bullseye_options,
+                                             bullseye_stats, load_library)
+    cfg = synth.CONFIGS["c2"]
+    im = cfg["image"]
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    sim = dict(cfg["sim"])
+    sim["n_time"] = args.batch_timesteps
+    obs = synth.simulate(**sim, seed=args.seed, device=dev)
+    wmax = float(np.float32(np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :]).max()))
+    plane_counts = [16, 24, 32, 48, 64, 96, 128]
+    ksets = []
+    for P in plane_counts:
+        halves = kernels.linear_plane_supports(P, 3, im["support"])
+        conv = kernels.wproj_kernels(im["support"], im["oversample"], P, wmax, im["npix"],
+                                     im["cell"], half_supports=halves, dtype=np.complex64)
+        ksets.append((conv, halves))
+    rng = np.random.default_rng(args.seed + 200)
+    nf = args.batch_facets
+    sizes = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), nf)) / 2) * 2).astype(int)
+    kid = rng.integers(0, len(plane_counts), nf)
+    psf = np.zeros(nf, bool)
+    psf[rng.choice(nf, nf // 4, replace=False)] = True
+    cen = synth.facet_centres(nf, 2.0, obs.phase_centre, rng)
+    facets = [BatchFacet(int(sizes[i]), im["cell"], (float(cen[i, 0]), float(cen[i, 1])),
+                         int(kid[i]), bool(psf[i])) for i in range(nf)]
+    log("synthesised %d rows x %d ch, %d kernel stacks, %d facets (%.0f Mpix) in %.1fs" % (
+        obs.rows, obs.n_chan, len(ksets), nf, float((sizes.astype(float) ** 2).sum()) / 1e6,
+        time.time() - t0))
+
+    lib = load_library("GPU", "mixed")
+
+    def check(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (what, lib.bullseye_last_error().decode()))
+
+    def set_opts(path):
+        o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=1,
+                             compute_path=path)
+        check(lib.bullseye_set_options(ctypes.byref(o)), "options")
+
+    spec0 = ImagingSpec(npix=256, cell=im["cell"], conv=kernels.pswf_aa_kernel(3, 63),
+                        support=3, oversample=63)
+    b = Binding("mixed", obs, spec0, device=dev, with_psf=True)
+    torch.cuda.synchronize()
+    set_opts(PATH_FP32_KAHAN)
+    check(lib.initLibrary(b.byref()), "initLibrary")
+    batch = FacetBatch(lib, b, "facet_single_pol")
+    for k, (conv, halves) in enumerate(ksets):
+        batch.kernel_set(k, conv, im["support"], im["oversample"], halves, wmax)
+    # the CPU baseline's facet: the median size among the visibility facets
+    vis_idx = [i for i in range(nf) if not psf[i]]
+    cpu_f = facets[sorted(vis_idx, key=lambda i: sizes[i])[len(vis_idx) // 2]]
+    cpu_conv, cpu_halves = ksets[cpu_f.kernel_set]
+    del ksets
+    stats = bullseye_stats()
+    calibration_copy(args.calibration_copy_mib)
+
+    tune = None
+    if args.tuning:
+        tj = json.loads(args.tuning)
+        batch.set_tuning(channels_per_stream=tj.get("channels_per_stream", 0),
+                         base_rows=tj.get("base_rows", 0),
+                         shape={int(m): v for m, v in tj.get("shape", {}).items()},
+                         chunk={int(m): v for m, v in tj.get("chunk", {}).items()})
+    elif args.tune_timesteps > 0:
+        t = time.time()
+        bt_ = Binding("mixed", obs, spec0, rows=(0, args.tune_timesteps * obs.baseline_count),
+                      device=dev, with_psf=True)
+        tuner = FacetBatch(lib, bt_, "facet_single_pol")
+        tuner.set_tuning(channels_per_stream=8)
+        tune = tuner.autotune(facets)
+        del bt_, tuner
+        log("autotune on %d timesteps: %.1fs, tuned %.1f ms vs default tiles %.1f ms; cps %d, "
+            "base rows %d" % (args.tune_timesteps, time.time() - t, tune["tuned_ms"],
+                              tune["default_ms"], tune["channels_per_stream"], tune["base_rows"]))
+
+    def run(path, schedule, steps, warmup, what):
+        set_opts(path)
+        t = time.time()
+        batch.plan(facets, schedule)
+        tplan = time.time() - t
+        for _ in range(warmup):
+            batch.execute()
+            log("%s: warmup step done" % what)
+        torch.cuda.synchronize()
+        kms, taps = [], 0
+        t = time.perf_counter()
+        for i in range(steps):
+            batch.execute()
+            lib.bullseye_get_stats(ctypes.byref(stats))
+            kms.append(stats.kernel_ms)
+            taps = stats.taps
+            log("%s: step %d, kernels %.1f ms" % (what, i, stats.kernel_ms))
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / steps
+        log("%s: %.1f ms/step (plan %.1fs), %.3e taps, %.1f Gtaps/s" % (
+            what, wall * 1e3, tplan, taps, taps / wall / 1e9))
+        return wall, taps, float(np.mean(kms)), stats.samples_gridded, tplan
+
+    wall32, taps, k32, nsamp, tplan = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, args.steps,
+                                          args.warmup, "persistent graph, tuned")
+    tuned = batch.tuning()
+    compare = {}
+    if not args.no_compare:
+        out32 = [o.clone() for o in batch.outputs]
+        wall64, taps64, _, _, _ = run(PATH_FP64, BATCH_PERSISTENT_GRAPH, 1, 0, "FP64 persistent")
+        errs = []
+        for o32, o64 in zip(out32, batch.outputs):
+            w = o64.to(torch.complex128)
+            d = (o32.to(torch.complex128) - w).abs()
+            peak = w.abs().max().item()
+            if peak > 0:
+                errs.append((torch.sqrt((d ** 2).mean()).item() / peak, d.max().item() / peak))
+        del out32
+        batch.set_tuning()
+        wall_d, taps_d, _, _, _ = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, 1, 0,
+                                      "persistent graph, default tiles")
+        wall_pf, taps_pf, _, _, _ = run(PATH_FP32_KAHAN, BATCH_PER_FACET, 1, 0,
+                                        "one launch per facet and class")
+        assert taps_d == taps and taps_pf == taps and taps64 == taps
+        compare = {
+            "fp64_gtaps_per_s": round(taps64 / wall64 / 1e9, 3),
+            "fp32comp_over_fp64": round(wall64 / wall32, 3),
+            "default_tiles_gtaps_per_s": round(taps_d / wall_d / 1e9, 3),
+            "per_facet_launch_gtaps_per_s": round(taps_pf / wall_pf / 1e9, 3),
+            "batched_over_per_facet": round(wall_pf / wall32, 3),
+            "rms_err_vs_fp64_gpu_worst_facet": max(e[0] for e in errs),
+            "peak_err_vs_fp64_gpu_worst_facet": max(e[1] for e in errs)}
+    batch.release()
+    lib.releaseLibrary()
+    flops = taps * 8
+    n_psf = int(psf.sum())
+    result = {
+        "metric": METRIC, "value": round(taps / wall32 / 1e9, 3), "unit": "Gtaps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall32 * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded; inputs resident in HBM)",
+        "config": {"workload": "batched uneven facets: w-projection visibilities (128 ant, 8128 "
+                               "baselines, %d t x 32 ch 1.0-1.5 GHz, 1 corr) into %d facets, "
+                               "sizes log-uniform 128^2-2048^2 at 1.5 arcsec, 7 w-kernel stacks "
+                               "of 16-128 planes (supports 7->63 by |w|, OS x8), %d sampling-"
+                               "function facets; persistent class launches in one HIP graph"
+                               % (args.batch_timesteps, nf, n_psf),
+                   "rows": obs.rows, "channels": obs.n_chan, "facets": nf,
+                   "facet_mpix": round(float((sizes.astype(float) ** 2).sum()) / 1e6, 2),
+                   "taps_per_step": taps, "samples_per_step": nsamp,
+                   "plan_s": round(tplan, 2),
+                   "tuning": {"channels_per_stream": tuned["channels_per_stream"],
+                              "base_rows": tuned["base_rows"],
+                              "shape": {m: v for m, v in tuned["shape"].items() if v},
+                              "chunk": {m: v for m, v in tuned["chunk"].items() if v > 1}}},
+        **compare,
+        "roofline": {"bound": "mfma", "achieved": round(flops / (k32 * 1e-3) / 1e12, 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("batch_facets_fp32comp_pmc.md"),
+                     "kernel_ms": round(k32, 3), "flops_per_launch": flops},
+    }
+    if tune is not None:
+        result["autotune"] = {"timesteps": args.tune_timesteps,
+                              "tuned_ms": round(tune["tuned_ms"], 3),
+                              "default_ms": round(tune["default_ms"], 3)}
+    if not args.no_cpu:
+        # one facet of the batch on the first two timesteps: its GPU tap count through the
+        # single-facet entry point, then the oracle's single-precision build on the same rows
+        from oracle import oracle
+        P = plane_counts[cpu_f.kernel_set]
+        spec_f = ImagingSpec(npix=cpu_f.npix, cell=cpu_f.cell, conv=cpu_conv,
+                             support=im["support"], oversample=im["oversample"], wplanes=P,
+                             wmax=wmax, facet_centres=np.array([cpu_f.centre]))
+        sub = (0, 2 * obs.baseline_count)
+        arr = (ctypes.c_uint32 * len(cpu_halves))(*cpu_halves)
+        check(lib.bullseye_set_wplane_support(arr, len(cpu_halves)), "supports")
+        set_opts(PATH_FP32_KAHAN)
+        bs = Binding("mixed", obs, spec_f, rows=sub, device=dev)
+        torch.cuda.synchronize()
+        check(lib.initLibrary(bs.byref()), "initLibrary(sample)")
+        check(lib.facet_single_pol(bs.byref()), "grid(sample)")
+        lib.gridding_barrier()
+        lib.bullseye_get_stats(ctypes.byref(stats))
+        sample_taps = stats.taps
+        lib.releaseLibrary()
+        lib.bullseye_set_wplane_support(None, 0)
+        del bs
+        ref32 = Binding("single", obs, spec_f, rows=sub)
+        t = time.perf_counter()
+        oracle.run(ref32, "facet_single_pol", precision="single", normalize=False)
+        t_or32 = time.perf_counter() - t
+        result["cpu_baseline"] = {
+            "value": round(sample_taps / t_or32 / 1e9, 4), "unit": "Gtaps/s", "cores": 1,
+            "kind": "port",
+            "sample": "one %d^2 facet of the batch (%d w-planes) on the first 2 of %d timesteps, "
+                      "%.2e in-support taps, %.1f s; one thread (the reference parallelises over "
+                      "the facets of one geometry); the oracle visits the full 63x63 support"
+                      % (cpu_f.npix, P, args.batch_timesteps, sample_taps, t_or32)}
     print(json.dumps(result), flush=True)
 
 
