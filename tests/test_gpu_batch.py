@@ -62,20 +62,33 @@ def facet_list(obs, precision, facets=FACETS, seed=2):
             for i, (n, cell, kid, psf) in enumerate(facets)]
 
 
-def single_facet_spec(kspec, f):
+NC = {"facet_single_pol": 1, "facet_duel_pol": 2, "facet_4_cor": 4}
+
+
+def single_facet_spec(kspec, f, mode="facet_single_pol", cube=1):
     spec = kspec.__class__(**kspec.__dict__)
     spec.npix, spec.cell = f.npix, f.cell
     spec.facet_centres = np.array([f.centre], np.float64)
-    spec.n_pol_gridded = 1
+    spec.n_pol_gridded = NC[mode]
+    spec.pol_index, spec.second_pol_index = (0, 3) if mode == "facet_duel_pol" else (0, 1)
+    spec.cube = cube
+    if cube > 1:
+        spec.channel_grid_index = CGI(cube)
+        spec.sf_channel_grid_index = CGI(cube)
     if f.sampling_function:
-        spec.sf_count = 1
+        spec.sf_count = cube
     return spec
 
 
-def single_facet(lib, precision, obs, kspec, halves, f):
-    """The facet through facet_single_pol / facet_sampling_function alone: (grid, norm, taps)."""
-    spec = single_facet_spec(kspec, f)
-    mode = "facet_sampling_function" if f.sampling_function else "facet_single_pol"
+def CGI(cube, n_chan=8):
+    return (np.arange(n_chan) % cube).astype(np.uint64)
+
+
+def single_facet(lib, precision, obs, kspec, halves, f, mode="facet_single_pol", cube=1):
+    """The facet through its single-facet entry point alone: (grid, norm, taps)."""
+    spec = single_facet_spec(kspec, f, mode, cube)
+    nnorm = cube * NC[mode]
+    mode = "facet_sampling_function" if f.sampling_function else mode
     arr = (ctypes.c_uint32 * len(halves))(*halves)
     assert lib.bullseye_set_wplane_support(arr, len(halves)) == 0
     b = Binding(precision, obs, spec, with_psf=f.sampling_function)
@@ -87,7 +100,7 @@ def single_facet(lib, precision, obs, kspec, halves, f):
         lib.bullseye_get_stats(ctypes.byref(st))
         norm = np.ctypeslib.as_array(
             ctypes.cast(b.params.normalization_terms, ctypes.POINTER(ctypes.c_double)),
-            (1,)).copy()
+            (nnorm,)).copy()
     finally:
         lib.releaseLibrary()
         lib.bullseye_set_wplane_support(None, 0)
@@ -98,9 +111,11 @@ def single_facet(lib, precision, obs, kspec, halves, f):
 class Session:
     """initLibrary on the visibility set (any geometry), kernel sets defined, batch ready."""
 
-    def __init__(self, lib, precision, path, obs, kspecs, device=DEV):
+    def __init__(self, lib, precision, path, obs, kspecs, device=DEV, mode="facet_single_pol",
+                 cube=1):
         self.lib, self.path = lib, path
-        spec0 = kspecs[0][0]
+        spec0 = single_facet_spec(kspecs[0][0], BatchFacet(128, 15.0, (0.0, 0.0), 0, True),
+                                  mode, cube)
         self.binding = Binding(precision, obs, spec0, device=device, with_psf=True)
         set_options(lib, device_resident=1 if device is not None else 0,
                     compute_path=self.path)
@@ -109,7 +124,7 @@ class Session:
         assert lib.bullseye_set_wplane_support(arr, len(halves0)) == 0
         assert lib.initLibrary(self.binding.byref()) == 0, lib.bullseye_last_error()
         lib.bullseye_set_wplane_support(None, 0)
-        self.batch = FacetBatch(lib, self.binding, "facet_single_pol")
+        self.batch = FacetBatch(lib, self.binding, mode)
         for kid, (spec, halves) in kspecs.items():
             self.batch.kernel_set(kid, spec.conv, spec.support, spec.oversample, halves,
                                   spec.wmax)
@@ -143,9 +158,9 @@ def check_close(got, want, path):
     return rms, peak
 
 
-def open_session(precision, path, obs, kspecs, device=DEV):
+def open_session(precision, path, obs, kspecs, device=DEV, mode="facet_single_pol", cube=1):
     lib = load_library("GPU", precision)
-    return lib, Session(lib, precision, path, obs, kspecs, device=device)
+    return lib, Session(lib, precision, path, obs, kspecs, device=device, mode=mode, cube=cube)
 
 
 @pytest.mark.parametrize("precision,path", [("double", PATH_FP64), ("mixed", PATH_FP32_KAHAN)])
@@ -334,3 +349,56 @@ def test_batch_rejects_bad_tables():
             s.batch.execute()
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("mode,cube", [("facet_duel_pol", 1), ("facet_4_cor", 1),
+                                       ("facet_single_pol", 2), ("facet_4_cor", 3)])
+def test_batch_modes_and_cubes_match_single_facet_calls(mode, cube):
+    """2- and 4-correlation facets, and channels spread over several grid slices (every facet's
+    output and sums [cube][ncorr]; sampling-function facets over the same slices)."""
+    print("case batch", mode, "cube", cube, flush=True)
+    precision, path = "mixed", PATH_FP32_KAHAN
+    obs = round_to(small_obs("facet_4_cor", seed=23, n_ant=12, n_time=30, n_chan=8), precision)
+    kspecs = kernel_specs(obs, precision)
+    facets = facet_list(obs, precision, facets=FACETS[:4], seed=16)
+    lib = load_library("GPU", precision)
+    set_options(lib, compute_path=path)
+    want = [single_facet(lib, precision, obs, *kspecs[f.kernel_set], f, mode, cube)
+            for f in facets]
+    lib, s = open_session(precision, path, obs, kspecs, mode=mode, cube=cube)
+    try:
+        grids, norms, taps = s.run(facets)
+    finally:
+        s.close()
+    for i, f in enumerate(facets):
+        assert grids[i].shape == want[i][0].shape, (grids[i].shape, want[i][0].shape)
+        rms, peak = check_close(grids[i], want[i][0], path)
+        print("facet %d: rms %.2e peak %.2e" % (i, rms, peak))
+        if not f.sampling_function:
+            np.testing.assert_allclose(norms[i].reshape(-1), want[i][1], rtol=1e-9)
+    assert taps == sum(w[2] for w in want)
+
+
+def test_batch_non_square_facet_matches_oracle():
+    """nx != ny and cell_x != cell_y: u scales with nx * cell_x, v with ny * cell_y."""
+    print("case batch non-square", flush=True)
+    precision, path = "double", PATH_FP64
+    obs = observation(precision, seed=25)
+    kspecs = kernel_specs(obs, precision)
+    f = facet_list(obs, precision, facets=[(128, 15.0, 0, False)], seed=18)[0]
+    f.ny, f.cell_y = 96, 19.0
+    lib, s = open_session(precision, path, obs, kspecs)
+    try:
+        grids, norms, _ = s.run([f])
+    finally:
+        s.close()
+    assert grids[0].shape == (1, 1, 96, 128)
+    spec = single_facet_spec(kspecs[0][0], f)
+    ref = Binding("double", obs, spec)
+    ref.params.ny = 96
+    ref.params.cell_size_y = 19.0
+    out = ref.set_array("output_buffer", np.zeros((1, 1, 1, 96, 128), np.complex128),
+                        np.complex128)
+    want_norm = oracle.run(ref, "facet_single_pol", normalize=False)
+    check_close(grids[0], out[0], path)
+    np.testing.assert_allclose(norms[0].reshape(-1), want_norm, rtol=1e-9)
