@@ -566,7 +566,8 @@ def main_batch(args):
         batch.set_tuning(channels_per_stream=tj.get("channels_per_stream", 0),
                          base_rows=tj.get("base_rows", 0),
                          shape={int(m): v for m, v in tj.get("shape", {}).items()},
-                         chunk={int(m): v for m, v in tj.get("chunk", {}).items()})
+                         chunk={int(m): v for m, v in tj.get("chunk", {}).items()},
+                         facet_major=tj.get("facet_major", 0))
     elif args.tune_timesteps > 0:
         t = time.time()
         bt_ = Binding("mixed", obs, spec0, rows=(0, args.tune_timesteps * obs.baseline_count),
@@ -652,6 +653,7 @@ def main_batch(args):
                    "plan_s": round(tplan, 2),
                    "tuning": {"channels_per_stream": tuned["channels_per_stream"],
                               "base_rows": tuned["base_rows"],
+                              "facet_major": tuned["facet_major"],
                               "shape": {m: v for m, v in tuned["shape"].items() if v},
                               "chunk": {m: v for m, v in tuned["chunk"].items() if v > 1}}},
         **compare,

@@ -116,9 +116,11 @@ class FacetBatch:
         self._check(self.lib.bullseye_batch_execute())
         return self.outputs, self.norms
 
-    def set_tuning(self, channels_per_stream=0, base_rows=0, shape=None, chunk=None):
+    def set_tuning(self, channels_per_stream=0, base_rows=0, shape=None, chunk=None,
+                   facet_major=0):
         t = bullseye_batch_tuning(struct_size=ctypes.sizeof(bullseye_batch_tuning),
-                                  channels_per_stream=channels_per_stream, base_rows=base_rows)
+                                  channels_per_stream=channels_per_stream, base_rows=base_rows,
+                                  facet_major=facet_major)
         for i in range(32):   # dict {class width M: value} or one value for every class
             t.shape[i] = _per_class(shape, i)
             t.chunk[i] = _per_class(chunk, i)
@@ -159,7 +161,7 @@ def _per_class(v, i):
 
 def tuning_dict(t):
     return {"channels_per_stream": int(t.channels_per_stream), "base_rows": int(t.base_rows),
-            "tuned": int(t.tuned),
+            "tuned": int(t.tuned), "facet_major": int(t.facet_major),
             "shape": {2 * i + 1: int(t.shape[i]) for i in range(32)},
             "chunk": {2 * i + 1: int(t.chunk[i]) for i in range(32)},
             "tuned_ms": float(t.tuned_ms), "default_ms": float(t.default_ms)}

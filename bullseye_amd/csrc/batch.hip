@@ -122,7 +122,7 @@ struct Plan {
 
 KernelSet sets[MAX_SETS];
 Plan plan;
-bullseye_batch_tuning tuning = {sizeof(bullseye_batch_tuning), 0, 0, 0, {0}, {0}, 0, 0};
+bullseye_batch_tuning tuning = {sizeof(bullseye_batch_tuning), 0, 0, 0, {0}, {0}, 0, 0, 0, 0};
 
 int mode_of(int grid_mode) {
     switch (grid_mode) {
@@ -295,6 +295,15 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         }
         for (uint32_t i = 0; i < NCLS; ++i) {
             const uint64_t off = pl.lists_host.size();
+            if (t.facet_major) {  // each facet's streams together
+                for (uint32_t f = 0; f < pl.n; ++f) {
+                    if (pl.fkind[f] != kind) continue;
+                    for (uint32_t id : streams[pl.set_slot[pl.fset[f]]][i])
+                        pl.lists_host.push_back((f << 24) | id);
+                }
+                if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+                continue;
+            }
             // merge the kernel sets' ascending stream lists; each stream's facets together
             std::vector<size_t> pos(nsets_used, 0);
             for (;;) {
@@ -814,6 +823,21 @@ int autotune_impl(gridding_parameters *p, int mode, const bullseye_batch_facet *
                 if (base == 32 || base_list.size() == 1) default_total = dtotal;
         }
     }
+    // work-list order at the chosen tiles
+    double order_ms[2] = {0, 0};
+    for (uint32_t fm = 0; fm < 2; ++fm) {
+        bullseye_batch_tuning t = best_t;
+        t.facet_major = fm;
+        if (build_plan(p, mode, facets, n, BULLSEYE_BATCH_PERSISTENT, t, plan)) return 1;
+        for (size_t j = 0; j < plan.launches.size(); ++j) {
+            double ms = 0;
+            if (time_launch(plan, j, plan.launches[j].shape, ms)) return 1;
+            order_ms[fm] += ms;
+        }
+    }
+    if (check_device_errors()) return 1;
+    best_t.facet_major = order_ms[1] < order_ms[0] ? 1 : 0;
+    best_total = std::min(order_ms[0], order_ms[1]);
     plan.release();
     best_t.tuned = 1;
     best_t.tuned_ms = best_total;

@@ -76,6 +76,8 @@ class bullseye_batch_tuning(ctypes.Structure):
         ("tuned", ctypes.c_uint32),
         ("shape", ctypes.c_uint32 * 32),
         ("chunk", ctypes.c_uint32 * 32),
+        ("facet_major", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
         ("tuned_ms", ctypes.c_double),
         ("default_ms", ctypes.c_double),
     ]

@@ -340,6 +340,10 @@ typedef struct bullseye_batch_tuning {
     uint32_t tuned;             /* 1 when filled by bullseye_batch_autotune */
     uint32_t shape[32];
     uint32_t chunk[32];
+    uint32_t facet_major;       /* work-list order: 0 = the facets of a stream together (the
+                                   stream's visibilities shared in L2), 1 = each facet's streams
+                                   together (its grid's atomics local in the memory-side cache) */
+    uint32_t reserved;
     double tuned_ms;            /* autotune: summed device time of the chosen launches */
     double default_ms;          /* autotune: the same with the default tiles */
 } bullseye_batch_tuning;
@@ -360,7 +364,8 @@ int bullseye_batch_plan(gridding_parameters *params, int mode, const bullseye_ba
  * bullseye_stats (kernel_ms = device time of the gridding launches).  Synchronous. */
 int bullseye_batch_execute(void);
 /* Time every support class of the batch under each lane shape and chunk length, and each
- * channels-per-stream / base-rows pair, keep the fastest (as bullseye_batch_set_tuning) and
+ * channels-per-stream / base-rows pair (those left 0 in the current tuning), then the two
+ * work-list orders at the chosen tiles; keep the fastest (as bullseye_batch_set_tuning) and
  * return it.  Leaves no plan: call bullseye_batch_plan afterwards. */
 int bullseye_batch_autotune(gridding_parameters *params, int mode,
                             const bullseye_batch_facet *facets, uint32_t n,

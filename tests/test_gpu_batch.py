@@ -242,7 +242,7 @@ def test_batch_schedules_and_graph_replays_agree():
 TILES = [dict(chunk=2), dict(chunk=4), dict(shape=SHAPE_PAIRS1), dict(shape=SHAPE_PAIRS2),
          dict(shape=SHAPE_PAIRS4, chunk=2), dict(shape=SHAPE_COLUMN),
          dict(channels_per_stream=2, base_rows=16), dict(channels_per_stream=1, base_rows=8),
-         dict(base_rows=64, chunk=4)]
+         dict(base_rows=64, chunk=4), dict(facet_major=1), dict(facet_major=1, chunk=2)]
 
 
 @pytest.mark.parametrize("tile", range(len(TILES)))
