@@ -208,7 +208,9 @@ template <int MODE, int PATH> struct WPrep {
     }
 };
 
-template <int MODE, int PATH, int PAIRS> struct WGridder {
+// BATCH: list entries name their facet (WArgs::facet_args); without it every argument comes
+// from the kernel's own (uniform) arguments.
+template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
     typedef typename PathTraits<PATH>::acc_t T;
     static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
@@ -261,7 +263,18 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         const uint64_t li = item * w.streams_per_block + slot;
         const bool live = in_slot && li < w.n_list;
         uint32_t stream = 0, fidx = 0;
-        const WArgs &wf = live ? entry_args(w, li, stream, fidx) : w;
+        const WArgs *wfp = &w;
+        if constexpr (BATCH) {
+            // a stream slot is whole wavefronts, so the entry is uniform across the wave
+            uint32_t e = live ? w.stream_list[li] : 0u;
+            e = __builtin_amdgcn_readfirstlane(e);
+            fidx = e >> 24;
+            stream = e & 0xffffffu;
+            wfp = w.facet_args + fidx;
+        } else {
+            if (live) stream = w.stream_list[li];
+        }
+        const WArgs &wf = *wfp;
         const GridArgs &a = wf.g;
         R_t *recs = reinterpret_cast<R_t *>(smem) + slot * REC;
         __shared__ uint32_t s_steps;
@@ -320,7 +333,7 @@ template <int MODE, int PATH, int PAIRS> struct WGridder {
         unsigned long long cnt = 0, taps = 0;
         const cpx<T> *tapbase = reinterpret_cast<const cpx<T> *>(wf.taps);
         // normalisation key: facet slice, or the global facet of a batched entry
-        const int kbase = w.facet_args ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
+        const int kbase = BATCH ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         for (uint32_t st = 0; st < steps; ++st) {
             const uint32_t step0 = st * REC;
@@ -410,7 +423,7 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int MODE, int PATH, int M> struct WColGridder {
+template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
     static constexpr int CPL = NC == 4 ? (PATH == PATH_FP64 ? 2 : 1) : NC;
     static constexpr int NSPLIT = NC / CPL;
@@ -586,8 +599,17 @@ template <int MODE, int PATH, int M> struct WColGridder {
         if (gw >= GPW) return;
         const uint64_t li = it * (uint64_t)GPW + gw;
         if (li >= w.n_list) return;
-        uint32_t stream, fidx;
-        const WArgs &wf = entry_args(w, li, stream, fidx);
+        uint32_t stream, fidx = 0;
+        const WArgs *wfp = &w;
+        if constexpr (BATCH) {
+            const uint32_t e = w.stream_list[li];
+            fidx = e >> 24;
+            stream = e & 0xffffffu;
+            wfp = w.facet_args + fidx;
+        } else {
+            stream = w.stream_list[li];
+        }
+        const WArgs &wf = *wfp;
         const GridArgs &a = wf.g;
         uint64_t t = stream;
         const uint32_t facet = (uint32_t)(t % a.facets);
@@ -606,7 +628,7 @@ template <int MODE, int PATH, int M> struct WColGridder {
         const uint32_t nrows = r_hi > r_lo ? r_hi - r_lo : 0;
         const uint32_t nsamp = nrows * ncb;
         const bool tin = *a.time_inner != 0;
-        const int kbase = w.facet_args ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
+        const int kbase = BATCH ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         const int ua = gl % M;          // u residue of this lane
         const int c0 = (gl / M) * CPL;  // first correlation of this lane
@@ -709,10 +731,10 @@ template <int MODE, int PATH, int M> struct WColGridder {
     }
 };
 
-template <int MODE, int PATH, int M>
+template <int MODE, int PATH, int M, bool BATCH>
 __global__ __launch_bounds__(WCOL_BLOCK) void wcol_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    typedef WColGridder<MODE, PATH, M> Gd;
+    typedef WColGridder<MODE, PATH, M, BATCH> Gd;
     Gd::run(w, smem + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
@@ -734,76 +756,77 @@ static uint64_t persistent_blocks(K kernel, uint32_t block, size_t smem, uint64_
     return std::min<uint64_t>(needed, (uint64_t)per * (uint64_t)cus);
 }
 
-template <int MODE, int PATH, int M>
+template <int MODE, int PATH, int M, bool BATCH>
 static hipError_t launch_col(const WArgs &w, hipStream_t st) {
-    typedef WColGridder<MODE, PATH, M> Gd;
+    typedef WColGridder<MODE, PATH, M, BATCH> Gd;
     const uint64_t waves = (w.n_list + Gd::GPW - 1) / Gd::GPW;
     uint64_t blocks = (waves + WCOL_BLOCK / 64 - 1) / (WCOL_BLOCK / 64);
     if (!blocks) return hipSuccess;
     const size_t smem = (size_t)(WCOL_BLOCK / 64) * Gd::WAVE_BYTES;
     if (w.queue)
-        blocks = persistent_blocks(wcol_kernel<MODE, PATH, M>, WCOL_BLOCK, smem, blocks);
-    wcol_kernel<MODE, PATH, M><<<dim3((uint32_t)blocks), dim3(WCOL_BLOCK), smem, st>>>(w);
+        blocks = persistent_blocks(wcol_kernel<MODE, PATH, M, BATCH>, WCOL_BLOCK, smem, blocks);
+    wcol_kernel<MODE, PATH, M, BATCH><<<dim3((uint32_t)blocks), dim3(WCOL_BLOCK), smem, st>>>(w);
     return hipGetLastError();
 }
 
-template <int MODE, int PATH>
+template <int MODE, int PATH, bool BATCH>
 static hipError_t launch_col_m(const WArgs &w, hipStream_t st) {
     switch (w.M) {
-        case 1: return launch_col<MODE, PATH, 1>(w, st);
-        case 3: return launch_col<MODE, PATH, 3>(w, st);
-        case 5: return launch_col<MODE, PATH, 5>(w, st);
-        case 7: return launch_col<MODE, PATH, 7>(w, st);
-        case 9: return launch_col<MODE, PATH, 9>(w, st);
-        case 11: return launch_col<MODE, PATH, 11>(w, st);
-        case 13: return launch_col<MODE, PATH, 13>(w, st);
-        case 15: return launch_col<MODE, PATH, 15>(w, st);
+        case 1: return launch_col<MODE, PATH, 1, BATCH>(w, st);
+        case 3: return launch_col<MODE, PATH, 3, BATCH>(w, st);
+        case 5: return launch_col<MODE, PATH, 5, BATCH>(w, st);
+        case 7: return launch_col<MODE, PATH, 7, BATCH>(w, st);
+        case 9: return launch_col<MODE, PATH, 9, BATCH>(w, st);
+        case 11: return launch_col<MODE, PATH, 11, BATCH>(w, st);
+        case 13: return launch_col<MODE, PATH, 13, BATCH>(w, st);
+        case 15: return launch_col<MODE, PATH, 15, BATCH>(w, st);
     }
     return hipErrorInvalidValue;
 }
 
-template <int MODE, int PATH, int PAIRS>
+template <int MODE, int PATH, int PAIRS, bool BATCH>
 __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    WGridder<MODE, PATH, PAIRS>::run(w, smem);
+    WGridder<MODE, PATH, PAIRS, BATCH>::run(w, smem);
 }
 
-template <int MODE, int PATH, int PAIRS>
+template <int MODE, int PATH, int PAIRS, bool BATCH>
 static hipError_t launch_t(const WArgs &w, uint32_t block, hipStream_t st) {
-    typedef typename WGridder<MODE, PATH, PAIRS>::R_t R_t;
+    typedef typename WGridder<MODE, PATH, PAIRS, BATCH>::R_t R_t;
     uint64_t blocks = (w.n_list + w.streams_per_block - 1) / w.streams_per_block;
     if (!blocks) return hipSuccess;
     const size_t smem = (size_t)w.streams_per_block * REC * sizeof(R_t);
-    if (w.queue) blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS>, block, smem, blocks);
-    wproj_kernel<MODE, PATH, PAIRS><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
+    if (w.queue)
+        blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, BATCH>, block, smem, blocks);
+    wproj_kernel<MODE, PATH, PAIRS, BATCH><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
     return hipGetLastError();
 }
 
-template <int MODE, int PATH>
+template <int MODE, int PATH, bool BATCH>
 static hipError_t launch_p(const WArgs &w, int pairs, uint32_t block, hipStream_t st) {
     switch (pairs) {
-        case 1: return launch_t<MODE, PATH, 1>(w, block, st);
-        case 2: return launch_t<MODE, PATH, 2>(w, block, st);
-        case 4: return launch_t<MODE, PATH, 4>(w, block, st);
+        case 1: return launch_t<MODE, PATH, 1, BATCH>(w, block, st);
+        case 2: return launch_t<MODE, PATH, 2, BATCH>(w, block, st);
+        case 4: return launch_t<MODE, PATH, 4, BATCH>(w, block, st);
     }
     return hipErrorInvalidValue;
 }
 
-template <int MODE>
+template <int MODE, bool BATCH>
 static hipError_t launch_m(const WArgs &w, int path, bool column, int pairs, uint32_t block,
                            hipStream_t st) {
     if (column) {
         switch (path) {
-            case PATH_FP64: return launch_col_m<MODE, PATH_FP64>(w, st);
-            case PATH_F32K: return launch_col_m<MODE, PATH_F32K>(w, st);
-            case PATH_F32: return launch_col_m<MODE, PATH_F32>(w, st);
+            case PATH_FP64: return launch_col_m<MODE, PATH_FP64, BATCH>(w, st);
+            case PATH_F32K: return launch_col_m<MODE, PATH_F32K, BATCH>(w, st);
+            case PATH_F32: return launch_col_m<MODE, PATH_F32, BATCH>(w, st);
         }
         return hipErrorInvalidValue;
     }
     switch (path) {
-        case PATH_FP64: return launch_p<MODE, PATH_FP64>(w, pairs, block, st);
-        case PATH_F32K: return launch_p<MODE, PATH_F32K>(w, pairs, block, st);
-        case PATH_F32: return launch_p<MODE, PATH_F32>(w, pairs, block, st);
+        case PATH_FP64: return launch_p<MODE, PATH_FP64, BATCH>(w, pairs, block, st);
+        case PATH_F32K: return launch_p<MODE, PATH_F32K, BATCH>(w, pairs, block, st);
+        case PATH_F32: return launch_p<MODE, PATH_F32, BATCH>(w, pairs, block, st);
     }
     return hipErrorInvalidValue;
 }
@@ -877,11 +900,21 @@ hipError_t launch_wproj_class(const wp::WArgs &w0, int mode, int path, int shape
     wproj_shape((int)w.M, shape_pairs(shape), block, w.lanes_per_stream, w.streams_per_block,
                 pairs);
     switch (mode) {
-        case MODE_1: return wp::launch_m<MODE_1>(w, path, column, pairs, block, st);
-        case MODE_2: return wp::launch_m<MODE_2>(w, path, column, pairs, block, st);
-        case MODE_4: return wp::launch_m<MODE_4>(w, path, column, pairs, block, st);
-        case MODE_4J: return wp::launch_m<MODE_4J>(w, path, column, pairs, block, st);
-        case MODE_PSF: return wp::launch_m<MODE_PSF>(w, path, column, pairs, block, st);
+        case MODE_1:
+            return w.facet_args ? wp::launch_m<MODE_1, true>(w, path, column, pairs, block, st)
+                                : wp::launch_m<MODE_1, false>(w, path, column, pairs, block, st);
+        case MODE_2:
+            return w.facet_args ? wp::launch_m<MODE_2, true>(w, path, column, pairs, block, st)
+                                : wp::launch_m<MODE_2, false>(w, path, column, pairs, block, st);
+        case MODE_4:
+            return w.facet_args ? wp::launch_m<MODE_4, true>(w, path, column, pairs, block, st)
+                                : wp::launch_m<MODE_4, false>(w, path, column, pairs, block, st);
+        case MODE_4J:
+            return w.facet_args ? wp::launch_m<MODE_4J, true>(w, path, column, pairs, block, st)
+                                : wp::launch_m<MODE_4J, false>(w, path, column, pairs, block, st);
+        case MODE_PSF:
+            return w.facet_args ? wp::launch_m<MODE_PSF, true>(w, path, column, pairs, block, st)
+                                : wp::launch_m<MODE_PSF, false>(w, path, column, pairs, block, st);
     }
     return hipErrorInvalidValue;
 }

@@ -35,20 +35,6 @@ struct WArgs {
     uint32_t n_items;
 };
 
-// The arguments and stream index of list entry li (see WArgs::facet_args).
-__device__ __forceinline__ const WArgs &entry_args(const WArgs &w, uint64_t li, uint32_t &stream,
-                                                   uint32_t &fidx) {
-    const uint32_t e = w.stream_list[li];
-    if (w.facet_args) {
-        fidx = e >> 24;
-        stream = e & 0xffffffu;
-        return w.facet_args[fidx];
-    }
-    fidx = 0;
-    stream = e;
-    return w;
-}
-
 }  // namespace wp
 
 // Lane shapes of a class launch: the column-ownership kernel (M <= 15), or the residue-pair
