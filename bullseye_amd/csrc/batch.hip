@@ -166,7 +166,9 @@ int validate(const LibView &lv, const gridding_parameters *p, int mode,
     if (!mode) return fail("batch mode must be facet_single_pol, facet_duel_pol or facet_4_cor");
     if (!p || !facets || n == 0) return fail("empty batch");
     if (n > MAX_FACETS) return fail("a batch holds at most %u facets", MAX_FACETS);
-    if (!p->antenna_1_ids || !p->antenna_2_ids) return fail("antenna_1_ids/antenna_2_ids are required");
+    const bool any_rows = p->row_count > 0;   // per-row arrays may be NULL in an empty set
+    if (any_rows && (!p->antenna_1_ids || !p->antenna_2_ids))
+        return fail("antenna_1_ids/antenna_2_ids are required");
     if (p->row_count > 0xffffffffull) return fail("row_count too large");
     const size_t npol = p->number_of_polarization_terms;
     if (mode == MODE_1 && p->polarization_index >= npol) return fail("polarization_index out of range");
@@ -187,13 +189,13 @@ int validate(const LibView &lv, const gridding_parameters *p, int mode,
         if (fc.sampling_function) any_psf = true;
         else any_vis = true;
     }
-    if (any_vis && (!p->visibilities || !p->visibility_weights || !p->flags))
+    if (any_rows && any_vis && (!p->visibilities || !p->visibility_weights || !p->flags))
         return fail("a required array pointer is NULL");
     if (any_vis && !p->channel_grid_indicies) return fail("channel_grid_indicies is NULL");
     if (any_psf && !p->sampling_function_channel_grid_indicies)
         return fail("sampling_function_channel_grid_indicies is NULL");
-    if (!p->uvw_coords || !p->reference_wavelengths || !p->flagged_rows || !p->field_array ||
-        !p->spw_index_array || !p->enabled_channels)
+    if (!p->reference_wavelengths || !p->enabled_channels ||
+        (any_rows && (!p->uvw_coords || !p->flagged_rows || !p->field_array || !p->spw_index_array)))
         return fail("a required array pointer is NULL");
     return 0;
 }

@@ -279,10 +279,13 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
                                 : p->channel_grid_indicies,
                   nspw * C * sizeof(size_t), &d));
     a.cgi = (const uint64_t *)d;
-    if (!a.uvw || !a.wavelengths || !a.row_flags || !a.field || !a.spw || !a.enabled || !a.cgi ||
-        (!psf && ((need_vis && !a.vis) || !a.weights || !a.flags)))
+    // per-row arrays may be NULL in an empty chunk (the reference's loops then never run)
+    const bool any_rows = rows > 0;
+    if (!a.wavelengths || !a.enabled || !a.cgi ||
+        (any_rows && (!a.uvw || !a.row_flags || !a.field || !a.spw ||
+                      (!psf && ((need_vis && !a.vis) || !a.weights || !a.flags)))))
         return fail("a required array pointer is NULL");
-    if (mode == MODE_4J) {
+    if (mode == MODE_4J && any_rows) {
         HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
         a.time_ids = (const uint64_t *)d;
         const size_t nj = p->no_timestamps_read * p->antenna_count * g.F * nspw * C;
@@ -374,9 +377,9 @@ static int validate_call(const gridding_parameters *p, int mode) {
         return fail("mode grids %d correlations but number_of_polarization_terms_being_gridded is %zu",
                     ncorr, g.P);
     if (psf && !g.psf) return fail("sampling function buffer was not set up at initLibrary");
-    if (!p->antenna_1_ids || !p->antenna_2_ids)
+    if (p->row_count && (!p->antenna_1_ids || !p->antenna_2_ids))
         return fail("antenna_1_ids/antenna_2_ids are required (Romein gridding walks baselines)");
-    if (mode == MODE_4J && !p->jones_terms) return fail("jones_terms is NULL");
+    if (mode == MODE_4J && p->row_count && !p->jones_terms) return fail("jones_terms is NULL");
     if (p->row_count > 0xffffffffull) return fail("row_count too large");
     const size_t npol = p->number_of_polarization_terms;
     if (mode == MODE_1) {
