@@ -217,7 +217,7 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle
-        n = args.cpu_sample_rows or (obs.rows // 4)
+        n = args.cpu_sample_rows or (obs.rows * 3 // 4)
         sub = (0, n)
         spec64 = ImagingSpec(**{**spec.__dict__, "jones": jones_reference_layout(
             obs, im["facets"], np.complex128)})
@@ -351,7 +351,7 @@ def main_wproj(args):
     }
     if not args.no_cpu:
         from oracle import oracle
-        n = args.cpu_sample_rows or 2 * 8128
+        n = args.cpu_sample_rows or 4 * 8128
         sub = (0, n)
         # GPU FP32-comp on the sample: its algorithmic tap count and its error vs the oracle
         lib.bullseye_set_wplane_support(arr, len(halves))
@@ -675,7 +675,7 @@ def main_batch(args):
         spec_f = ImagingSpec(npix=cpu_f.npix, cell=cpu_f.cell, conv=cpu_conv,
                              support=im["support"], oversample=im["oversample"], wplanes=P,
                              wmax=wmax, facet_centres=np.array([cpu_f.centre]))
-        sub = (0, 2 * obs.baseline_count)
+        sub = (0, 4 * obs.baseline_count)
         arr = (ctypes.c_uint32 * len(cpu_halves))(*cpu_halves)
         check(lib.bullseye_set_wplane_support(arr, len(cpu_halves)), "supports")
         set_opts(PATH_FP32_KAHAN)
@@ -696,7 +696,7 @@ def main_batch(args):
         result["cpu_baseline"] = {
             "value": round(sample_taps / t_or32 / 1e9, 4), "unit": "Gtaps/s", "cores": 1,
             "kind": "port",
-            "sample": "one %d^2 facet of the batch (%d w-planes) on the first 2 of %d timesteps, "
+            "sample": "one %d^2 facet of the batch (%d w-planes) on the first 4 of %d timesteps, "
                       "%.2e in-support taps, %.1f s; one thread (the reference parallelises over "
                       "the facets of one geometry); the oracle visits the full 63x63 support"
                       % (cpu_f.npix, P, args.batch_timesteps, sample_taps, t_or32)}
