@@ -111,8 +111,8 @@ def rel_errors(got, want):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: only --gpus 1 is supported (each box has one GPU; the facet-sharded "
-              "multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: only --gpus 1 is supported; the facet-sharded multi-GPU layout is a "
+              "design, not built", file=sys.stderr)
         sys.exit(2)
     if args.workload == "wproj":
         return main_wproj(args)
