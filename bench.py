@@ -13,7 +13,7 @@ device-resident buffers) and publish the grids to the caller's buffer.  One tap 
 each).  The FP64 path runs the same step on the same inputs; the error of the FP32-compensated
 grids is stated against it (full size) and against the FP64 CPU oracle (a row sample).
 
-Prints one JSON line.  --gpus N with N > 1 exits: every box here has one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the measurement is of one GPU.
 """
 import argparse
 import ctypes
