@@ -428,6 +428,7 @@ def main_adjoint(args):
     img = img.reshape(shape)
     log("synthesised %d rows x %d ch (%d samples), kernels in %.1fs" % (
         obs.rows, obs.n_chan, obs.rows * obs.n_chan, time.time() - t0))
+    calibration_copy(args.calibration_copy_mib)
 
     def step(im, image_t, data):
         image_t.grad = None
