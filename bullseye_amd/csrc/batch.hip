@@ -36,6 +36,12 @@ constexpr uint32_t MAX_SETS = 64;
 constexpr uint32_t MAX_FACETS = 255;
 constexpr int BRANCHES = 4;
 constexpr uint32_t NCLS = 32;   // class widths 1, 3, ..., 63
+constexpr uint32_t LIST_PAD = 0xffffffu;   // stream id of a no-op work-list entry
+
+// Streams per column-kernel item of class width M (1 where the class has no column kernel).
+uint32_t column_run(int M, int kind, int path) {
+    return M <= 15 ? wproj_streams_per_item(M, kind, path, WSHAPE_COLUMN) : 1u;
+}
 
 struct KernelSet {
     bool defined = false;
@@ -242,7 +248,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
     for (uint32_t i = 0; i < NCLS; ++i) {
         const uint32_t chunk = chunk_of(t, 2 * i + 1);
         const uint64_t nsegk = (pl.nseg_base + chunk - 1) / chunk;
-        if ((uint64_t)pl.B * pl.ncb * nsegk > 0xffffffull)
+        if ((uint64_t)pl.B * pl.ncb * nsegk >= LIST_PAD)
             return fail("batch too large: %llu streams per facet exceed 2^24",
                         (unsigned long long)((uint64_t)pl.B * pl.ncb * nsegk));
     }
@@ -281,6 +287,13 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         pl.launches.push_back(L);
         return 0;
     };
+    // Entries of one column-kernel item (the streams one wavefront walks, GPW of them) share a
+    // facet, so the kernel reads that facet's arguments once per wave with scalar loads; a
+    // short run is padded with no-op entries (stream PAD).
+    auto emit_run = [&](uint32_t f, const uint32_t *ids, size_t n, uint32_t g) {
+        for (size_t q = 0; q < n; ++q) pl.lists_host.push_back((f << 24) | ids[q]);
+        for (size_t q = n; q % g; ++q) pl.lists_host.push_back((f << 24) | LIST_PAD);
+    };
     for (int kind : {pl.mode, (int)MODE_PSF}) {
         auto &of = kind == MODE_PSF ? of_psf : of_vis;
         if (per_facet) {
@@ -289,7 +302,8 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                 const uint32_t k = pl.set_slot[pl.fset[f]];
                 for (uint32_t i = 0; i < NCLS; ++i) {
                     const uint64_t off = pl.lists_host.size();
-                    for (uint32_t id : streams[k][i]) pl.lists_host.push_back((f << 24) | id);
+                    const uint32_t g = column_run((int)(2 * i + 1), kind, pl.path);
+                    emit_run(f, streams[k][i].data(), streams[k][i].size(), g);
                     if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
                 }
             }
@@ -297,11 +311,31 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         }
         for (uint32_t i = 0; i < NCLS; ++i) {
             const uint64_t off = pl.lists_host.size();
+            const uint32_t g = column_run((int)(2 * i + 1), kind, pl.path);
+            if (g > 1 && !t.facet_major) {
+                // runs of g consecutive streams of a kernel set, every facet of the set in turn
+                // (the run's visibilities shared in L2), runs of all sets by first stream
+                std::vector<std::pair<uint32_t, uint32_t>> runs;   // (first stream, set)
+                for (uint32_t k = 0; k < nsets_used; ++k) {
+                    if (of[k].empty()) continue;
+                    for (size_t q = 0; q < streams[k][i].size(); q += g)
+                        runs.push_back({streams[k][i][q], k});
+                }
+                std::sort(runs.begin(), runs.end());
+                for (const auto &r : runs) {
+                    const std::vector<uint32_t> &sv = streams[r.second][i];
+                    const size_t q0 = std::lower_bound(sv.begin(), sv.end(), r.first) - sv.begin();
+                    const size_t n = std::min<size_t>(g, sv.size() - q0);
+                    for (uint32_t f : of[r.second]) emit_run(f, sv.data() + q0, n, g);
+                }
+                if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+                continue;
+            }
             if (t.facet_major) {  // each facet's streams together
                 for (uint32_t f = 0; f < pl.n; ++f) {
                     if (pl.fkind[f] != kind) continue;
-                    for (uint32_t id : streams[pl.set_slot[pl.fset[f]]][i])
-                        pl.lists_host.push_back((f << 24) | id);
+                    const std::vector<uint32_t> &sv = streams[pl.set_slot[pl.fset[f]]][i];
+                    emit_run(f, sv.data(), sv.size(), g);
                 }
                 if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
                 continue;
@@ -532,6 +566,11 @@ int build_plan(gridding_parameters *p, int mode, const bullseye_batch_facet *fac
     }
     a.phase = 1;
     a.transform = 1;
+    // the walk order compares footprint movement per time step and per channel step in grid
+    // cells; the ratio does not depend on the cell scale, so the first facet's scale stands in
+    // for all (the shared arguments carry no scale of their own)
+    a.u_scale = (double)((double)facets[0].nx * facets[0].cell_x * (double)ARCSEC_TO_RAD);
+    a.v_scale = -(double)((double)facets[0].ny * facets[0].cell_y * (double)ARCSEC_TO_RAD);
     HIP_TRY(pl.walk.ensure(sizeof(int)));
     a.time_inner = pl.walk.as<int>();
     HIP_TRY(launch_walk_order(a, lv.walk_order, pl.walk.as<int>(), lv.stream));

@@ -261,7 +261,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         const int sl = threadIdx.x - slot * L;       // lane within the stream
         const bool in_slot = slot < (int)w.streams_per_block;
         const uint64_t li = item * w.streams_per_block + slot;
-        const bool live = in_slot && li < w.n_list;
+        bool live = in_slot && li < w.n_list;
         uint32_t stream = 0, fidx = 0;
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
@@ -270,6 +270,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
             e = __builtin_amdgcn_readfirstlane(e);
             fidx = e >> 24;
             stream = e & 0xffffffu;
+            if (stream == 0xffffffu) stream = 0, live = false;   // padding entry
             wfp = w.facet_args + fidx;
         } else {
             if (live) stream = w.stream_list[li];
@@ -602,9 +603,10 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
         uint32_t stream, fidx = 0;
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
-            const uint32_t e = w.stream_list[li];
-            fidx = e >> 24;
-            stream = e & 0xffffffu;
+            // the host lays out an item's GPW entries with one facet: read it once per wave
+            fidx = __builtin_amdgcn_readfirstlane(w.stream_list[it * (uint64_t)GPW] >> 24);
+            stream = w.stream_list[li] & 0xffffffu;
+            if (stream == 0xffffffu) return;   // padding
             wfp = w.facet_args + fidx;
         } else {
             stream = w.stream_list[li];

@@ -98,6 +98,7 @@ def single_facet(lib, precision, obs, kspec, halves, f, mode="facet_single_pol",
         lib.gridding_barrier()
         st = bullseye_stats()
         lib.bullseye_get_stats(ctypes.byref(st))
+        single_facet.time_inner = int(st.time_inner)
         norm = np.ctypeslib.as_array(
             ctypes.cast(b.params.normalization_terms, ctypes.POINTER(ctypes.c_double)),
             (nnorm,)).copy()
@@ -402,3 +403,66 @@ def test_batch_non_square_facet_matches_oracle():
     want_norm = oracle.run(ref, "facet_single_pol", normalize=False)
     check_close(grids[0], out[0], path)
     np.testing.assert_allclose(norms[0].reshape(-1), want_norm, rtol=1e-9)
+
+
+def test_batch_holds_255_facets():
+    """The widest batch (facet index in 8 bits): 255 small facets of both kinds; every facet's
+    taps add up to those of its single-facet call (16 checked against the full grid), and a 256th
+    facet is refused."""
+    print("case batch 255 facets", flush=True)
+    precision, path = "double", PATH_FP64
+    obs = observation(precision, seed=27)
+    kspecs = kernel_specs(obs, precision)
+    rng = np.random.default_rng(20)
+    table = [(int(rng.choice([32, 48, 64])), float(rng.choice([20.0, 30.0, 40.0])),
+              int(rng.integers(0, 2)), bool(rng.random() < 0.25)) for _ in range(255)]
+    table = [(n, c, 0 if n < 40 else k, p) for n, c, k, p in table]  # set 1 needs > 2*12+4 px
+    facets = facet_list(obs, precision, facets=table, seed=22)
+    lib, s = open_session(precision, path, obs, kspecs)
+    try:
+        grids, norms, taps = s.run(facets)
+        with pytest.raises(RuntimeError, match="at most 255"):
+            s.batch.plan(facets + facets[:1])
+    finally:
+        s.close()
+    lib = load_library("GPU", precision)
+    set_options(lib, compute_path=path)
+    total = 0
+    for i, f in enumerate(facets):
+        want, wnorm, wtaps = single_facet(lib, precision, obs, *kspecs[f.kernel_set], f)
+        total += wtaps
+        if i % 16 == 0 and np.abs(want).max() > 0:
+            check_close(grids[i], want, path)
+            if not f.sampling_function:
+                np.testing.assert_allclose(norms[i].reshape(-1), wnorm, rtol=1e-9)
+    assert taps == total
+
+
+def test_batch_walk_order_matches_single_facet():
+    """The batch measures the walk order (time or channels inner) from the data as the
+    single-facet path does: the same choice on a narrow and on a wide band, and the two bands
+    choose differently."""
+    print("case batch walk order", flush=True)
+    precision, path = "double", PATH_FP64
+    orders = []
+    for f_lo, f_hi in ((1.40e9, 1.42e9), (1.0e9, 1.5e9)):
+        obs = round_to(small_obs("facet_single_pol", seed=29, n_ant=12, n_time=30, n_chan=8,
+                                 f_lo=f_lo, f_hi=f_hi), precision)
+        kspecs = kernel_specs(obs, precision)
+        f = facet_list(obs, precision, facets=[(128, 15.0, 0, False)], seed=24)[0]
+        lib = load_library("GPU", precision)
+        set_options(lib, compute_path=path)
+        single_facet(lib, precision, obs, *kspecs[0], f)
+        want = single_facet.time_inner
+        st = bullseye_stats()
+        lib, s = open_session(precision, path, obs, kspecs)
+        try:
+            s.run([f])
+            lib.bullseye_get_stats(ctypes.byref(st))
+            got = int(st.time_inner)
+        finally:
+            s.close()
+        print("band %.2f-%.2f GHz: single %d batch %d" % (f_lo / 1e9, f_hi / 1e9, want, got))
+        assert got == want
+        orders.append(got)
+    assert orders[0] != orders[1], orders
