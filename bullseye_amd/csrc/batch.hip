@@ -66,6 +66,7 @@ struct Launch {
     uint32_t M, chunk;
     int branch;
     double work;         // entries x M^2: the branch balancing estimate
+    bool single;         // one facet's launch of the single-facet kernels
 };
 
 struct Plan {
@@ -259,16 +260,24 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         (pl.fkind[f] == MODE_PSF ? of_psf : of_vis)[k].push_back(f);
     }
     const bool per_facet = pl.schedule == BULLSEYE_BATCH_PER_FACET;
-    auto add_launch = [&](int kind, uint32_t i, uint64_t off, uint64_t cnt) {
+    // facet < 0: a launch over the merged list of every facet; facet >= 0: a launch of the
+    // single-facet kernels with that facet's own arguments (plain stream ids, its normalisation
+    // block as the base of the sums)
+    auto add_launch = [&](int kind, uint32_t i, uint64_t off, uint64_t cnt, int facet) {
         if (!cnt) return 0;
         if (cnt > 0xffffffffull) return fail("work list too long");
         const uint32_t M = 2 * i + 1;
         const uint32_t chunk = chunk_of(t, M);
         Launch L;
         memset(&L.w, 0, sizeof L.w);
-        L.w.g = pl.base;
+        if (facet >= 0) {
+            L.w = pl.fargs_host[facet];
+            L.w.g.norm = pl.norm.as<double>() + (size_t)facet * pl.cubeB * pl.NC;
+        } else {
+            L.w.g = pl.base;
+            L.w.g.cube = pl.cubeB;
+        }
         L.w.g.n_segments = (pl.nseg_base + chunk - 1) / chunk;
-        L.w.g.cube = pl.cubeB;
         L.w.seg_rows = chunk * pl.base_rows;
         L.w.nseg = L.w.g.n_segments;
         L.w.base_rows = pl.base_rows;
@@ -284,6 +293,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         if (!wproj_shape_valid((int)M, L.shape)) L.shape = WSHAPE_DEFAULT;
         L.work = (double)cnt * M * M;
         L.branch = 0;
+        L.single = facet >= 0;
         pl.launches.push_back(L);
         return 0;
     };
@@ -302,9 +312,9 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                 const uint32_t k = pl.set_slot[pl.fset[f]];
                 for (uint32_t i = 0; i < NCLS; ++i) {
                     const uint64_t off = pl.lists_host.size();
-                    const uint32_t g = column_run((int)(2 * i + 1), kind, pl.path);
-                    emit_run(f, streams[k][i].data(), streams[k][i].size(), g);
-                    if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+                    pl.lists_host.insert(pl.lists_host.end(), streams[k][i].begin(),
+                                         streams[k][i].end());
+                    if (add_launch(kind, i, off, pl.lists_host.size() - off, (int)f)) return 1;
                 }
             }
             continue;
@@ -328,7 +338,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     const size_t n = std::min<size_t>(g, sv.size() - q0);
                     for (uint32_t f : of[r.second]) emit_run(f, sv.data() + q0, n, g);
                 }
-                if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+                if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
                 continue;
             }
             if (t.facet_major) {  // each facet's streams together
@@ -337,7 +347,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     const std::vector<uint32_t> &sv = streams[pl.set_slot[pl.fset[f]]][i];
                     emit_run(f, sv.data(), sv.size(), g);
                 }
-                if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+                if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
                 continue;
             }
             // merge the kernel sets' ascending stream lists; each stream's facets together
@@ -356,7 +366,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     ++pos[k];
                 }
             }
-            if (add_launch(kind, i, off, pl.lists_host.size() - off)) return 1;
+            if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
         }
     }
     // device lists, queues and branch assignment (longest first onto the least loaded)
@@ -381,7 +391,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
     for (size_t j = 0; j < pl.launches.size(); ++j) {
         Launch &L = pl.launches[j];
         L.w.stream_list = pl.lists.as<uint32_t>() + (uintptr_t)L.w.stream_list;
-        L.w.facet_args = pl.fargs.as<wp::WArgs>();
+        L.w.facet_args = L.single ? nullptr : pl.fargs.as<wp::WArgs>();
         L.w.queue = per_facet ? nullptr : pl.queues.as<uint32_t>() + j;
     }
     HIP_TRY(hipStreamSynchronize(lv.stream));  // lists_host stays valid until the upload is done
