@@ -188,6 +188,24 @@ def main():
     lib.releaseLibrary()
     rms_vs_gpu64, peak_vs_gpu64 = rel_errors(grid32.reshape(im["facets"], -1),
                                              grid64.reshape(im["facets"], -1))
+    # the reference's model: host arrays handed over every call (inputs copied to the device,
+    # grids copied back on the final chunk); reported beside `value`, never as it
+    bh = Binding("mixed", obs, spec)
+    o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=0,
+                         compute_path=PATH_FP32_KAHAN)
+    check(lib.bullseye_set_options(ctypes.byref(o)), "options")
+    check(lib.initLibrary(bh.byref()), "initLibrary(host)")
+    host_ms = []
+    for i in range(3):
+        t = time.perf_counter()
+        check(lib.bullseye_reset_grids(), "reset")
+        check(lib.facet_4_cor_corrections(bh.byref()), "grid(host)")
+        lib.gridding_barrier()
+        host_ms.append((time.perf_counter() - t) * 1e3)
+    lib.releaseLibrary()
+    del bh
+    wall_host = min(host_ms[1:]) * 1e-3
+    log("FP32-comp with host buffers (PCIe both ways): %.2f ms/step" % (wall_host * 1e3))
 
     gtaps32 = taps / wall32 / 1e9
     gtaps64 = taps64 / wall64 / 1e9
@@ -206,6 +224,8 @@ def main():
                    "facets": im["facets"], "npix": im["npix"], "taps_per_step": taps,
                    "samples_per_step": nsamp},
         "fp64_gtaps_per_s": round(gtaps64, 3), "fp32comp_over_fp64": round(gtaps32 / gtaps64, 3),
+        "host_buffers_ms_per_step": round(wall_host * 1e3, 3),
+        "host_buffers_gtaps_per_s": round(taps / wall_host / 1e9, 3),
         "rms_err_vs_fp64_gpu": rms_vs_gpu64, "peak_err_vs_fp64_gpu": peak_vs_gpu64,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
