@@ -466,3 +466,27 @@ def test_batch_walk_order_matches_single_facet():
         assert got == want
         orders.append(got)
     assert orders[0] != orders[1], orders
+
+
+@pytest.mark.parametrize("device", [DEV, None])
+def test_batch_of_an_empty_visibility_set(device):
+    """No rows: the plan has no work, every facet's grid and sums are zero."""
+    print("case batch empty", device, flush=True)
+    precision, path = "mixed", PATH_FP32_KAHAN
+    obs = observation(precision, seed=31)
+    kspecs = kernel_specs(obs, precision)
+    facets = facet_list(obs, precision, facets=FACETS[:3], seed=26)
+    lib = load_library("GPU", precision)
+    s = Session(lib, precision, path, obs, kspecs, device=device)
+    try:
+        empty = Binding(precision, obs, s.binding.spec, rows=(0, 0), device=device, with_psf=True)
+        batch = FacetBatch(lib, empty, "facet_single_pol")
+        batch.plan(facets)
+        grids, norms = batch.execute()
+        st = bullseye_stats()
+        lib.bullseye_get_stats(ctypes.byref(st))
+        assert st.taps == 0 and st.samples_gridded == 0
+        for g, n in zip(grids, norms):
+            assert np.abs(to_np(g)).max() == 0 and np.all(n == 0)
+    finally:
+        s.close()
