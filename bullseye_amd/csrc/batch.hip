@@ -395,6 +395,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         L.w.queue = per_facet ? nullptr : pl.queues.as<uint32_t>() + j;
     }
     HIP_TRY(hipStreamSynchronize(lv.stream));  // lists_host stays valid until the upload is done
+    std::vector<uint32_t>().swap(pl.lists_host);
     return 0;
 }
 

@@ -23,6 +23,7 @@ namespace bam {
 struct TimedCall {
     hipEvent_t start, k0, k1, stop;
     bool has_kernel;
+    bool recorded;   // every event recorded (a call that failed midway is only released)
 };
 
 struct State {
@@ -142,6 +143,11 @@ static int barrier_impl() {
     if (e != hipSuccess) return fail("gridding stream failed: %s", hipGetErrorString(e));
     for (auto &c : g.pending) {
         float ms = 0, kms = 0;
+        if (!c.recorded) {
+            for (hipEvent_t ev : {c.start, c.k0, c.k1, c.stop})
+                if (ev) (void)hipEventDestroy(ev);
+            continue;
+        }
         (void)hipEventElapsedTime(&ms, c.start, c.stop);
         g.grid_seconds += ms * 1e-3;
         g.stats.copy_ms = 0;
@@ -403,7 +409,9 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     const size_t nspw = std::max<size_t>(1, p->spw_count);
     const size_t B = p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
 
-    TimedCall tc{};
+    // registered before anything can fail, so the barrier releases the events of a failed call
+    g.pending.push_back(TimedCall{});
+    TimedCall &tc = g.pending.back();
     HIP_TRY(hipEventCreate(&tc.start));
     HIP_TRY(hipEventCreate(&tc.k0));
     HIP_TRY(hipEventCreate(&tc.k1));
@@ -517,7 +525,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     HIP_TRY(hipEventRecord(tc.k1, g.stream));
     tc.has_kernel = true;
     HIP_TRY(hipEventRecord(tc.stop, g.stream));
-    g.pending.push_back(tc);
+    tc.recorded = true;
     if (!dev_mode()) {
         // the caller may reuse its host arrays after gridding_barrier; staging copies are
         // complete once the stream reaches the kernel

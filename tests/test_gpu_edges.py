@@ -11,7 +11,7 @@ import pytest
 from common import (MODES_PSF, make_spec, rel_err, round_to, set_options, small_obs,
                     wproj_spec)
 from bullseye_amd.binding import Binding
-from bullseye_amd.library_loader import PATH_FP32_KAHAN, PATH_FP64, load_library
+from bullseye_amd.library_loader import PATH_FP32_KAHAN, PATH_FP64, bullseye_stats, load_library
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -171,3 +171,39 @@ def test_supports_beyond_the_kernels_are_rejected():
         assert b"exceeds 31" in lib.bullseye_last_error()
     finally:
         lib.releaseLibrary()
+
+
+def test_failed_call_leaves_the_library_usable():
+    """A call that fails midway (a NULL row array) reports the error; the next call of the same
+    pass grids normally and the barrier's timing and counters are unaffected."""
+    print("case failed call", flush=True)
+    precision, path = "mixed", PATH_FP32_KAHAN
+    mode = "facet_single_pol"
+    obs = round_to(small_obs(mode, seed=53), precision)
+    spec = make_spec(obs, mode, precision=precision)
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", precision)
+    set_options(lib, compute_path=path)
+    b = Binding(precision, obs, spec)
+    bad = Binding(precision, obs, spec)
+    bad.params.uvw_coords = None
+    try:
+        assert lib.initLibrary(b.byref()) == 0
+        bad.params.output_buffer = b.params.output_buffer
+        bad.params.is_final_data_chunk = False
+        assert lib.facet_single_pol(bad.byref()) != 0
+        assert b"NULL" in lib.bullseye_last_error()
+        assert lib.facet_single_pol(b.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        assert lib.bullseye_last_error() == b"" or b"NULL" in lib.bullseye_last_error()
+        st = bullseye_stats()
+        lib.bullseye_get_stats(ctypes.byref(st))
+        assert st.kernel_ms > 0 and st.taps > 0
+        assert lib.get_gridding_walltime() > 0
+        norm = np.ctypeslib.as_array(
+            ctypes.cast(b.params.normalization_terms, ctypes.POINTER(ctypes.c_double)),
+            (len(want_norm),)).copy()
+    finally:
+        lib.releaseLibrary()
+    check(b.output, want, precision, path)
+    np.testing.assert_allclose(norm, want_norm, rtol=1e-12)
