@@ -35,6 +35,30 @@ struct FacetGeom {
     double pad[3];
 };
 
+// One row of a Romein pass in baseline-sorted order, gathered once per call (through the sort
+// permutation) from the row arrays, so that a gridding kernel reads a row with one 64-byte load
+// instead of a chain of dependent gathers.  uvw is widened exactly to double.
+constexpr uint32_t ROW_USABLE = 1;    // not flagged and in the imaging field
+constexpr uint32_t ROW_BAD_IDS = 2;   // time or antenna id out of range (Jones modes)
+struct __align__(16) RowRec {
+    double u, v, w;
+    uint32_t row;          // original row index (visibility, weight and flag addressing)
+    uint32_t spw;          // spectral window as given (range-checked by the kernel)
+    uint32_t state;        // ROW_* bits
+    uint32_t pad;
+    uint64_t j1, j2;       // Jones modes: (t * antennas + ant) * facets * spw_count * chan
+};
+
+// One (spw, channel) of a pass: reciprocal wavelength (the same FP64 division a kernel would
+// do per sample), grid slice (clipped to 0xffffffff) and the enabled flag.
+struct __align__(16) ChanRec {
+    double rw;
+    uint32_t cgi;
+    uint32_t enabled;
+};
+static_assert(sizeof(RowRec) == 64 && sizeof(ChanRec) == 16, "table record sizes");
+constexpr uint32_t CHAN_LDS_MAX = 1024;  // channel tables up to this size are staged in LDS
+
 // One Romein stream: a lane group walks the rows [row_lo, row_hi) of baseline `baseline`
 // (in time order) for channels [chan_lo, chan_hi) of facet `facet`, serpentine over channels.
 // Streams are enumerated implicitly: index -> (facet, baseline, channel block, time segment).
@@ -60,6 +84,8 @@ struct GridArgs {
     const void *conv;                          // AA taps (acc_t) [csz]
     const double *tapsum;                      // [OS+1] sum of the 2S+1 taps per fraction
     const int *time_inner;                     // [1] walk order chosen on the device
+    const RowRec *rowrec;                      // [row] sorted rows (kernels that use them)
+    const ChanRec *chanrec;                    // [spw*chan]
     // outputs
     double *grid;                              // [facet][cube][ny][nx][pstride][2]
     double *norm;                              // [nslots][facet*cube*ncorr]
@@ -76,6 +102,8 @@ struct GridArgs {
     uint32_t nslots;                           // copies of norm and counters (>= 1)
     double u_scale, v_scale, off_x, off_y;
     int phase, transform;
+    uint32_t chan_lds;     // chanrec is staged in LDS (spw*chan <= CHAN_LDS_MAX)
+    uint32_t pol_contig;   // 4 correlations: a sample's vis/weights/flags are contiguous, aligned
 };
 
 // ---------- small complex helpers --------------------------------------------------------

@@ -150,6 +150,59 @@ __global__ __launch_bounds__(WALK_THREADS) void k_walk_order(GridArgs a, int ove
     }
 }
 
+// Sorted row table: row i of the pass is row perm[i] of the arrays.  Jones modes also range-check
+// the time and antenna ids here and precompute each antenna's Jones row offset
+// (t * antennas + ant) * facets * spw_count * chan; the gridding kernel raises the error for a
+// row with bad ids when one of its samples reaches the Jones correction, as it would have
+// reading the ids itself.
+__global__ void k_row_table(GridArgs a, uint32_t rows, int jones, uint64_t fsc, RowRec *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const uint32_t row = a.perm[i];
+    RowRec r;
+    r.u = (double)a.uvw[3 * (size_t)row];
+    r.v = (double)a.uvw[3 * (size_t)row + 1];
+    r.w = (double)a.uvw[3 * (size_t)row + 2];
+    r.row = row;
+    r.spw = a.spw[row];
+    r.state = (!a.row_flags[row] && a.field[row] == a.imaging_field) ? ROW_USABLE : 0u;
+    r.pad = 0;
+    r.j1 = r.j2 = 0;
+    if (jones) {
+        const uint64_t t = a.time_ids[row];
+        const uint32_t x = a.ant1[row], y = a.ant2[row];
+        if (t >= a.ntime || x >= a.antenna_count || y >= a.antenna_count) {
+            r.state |= ROW_BAD_IDS;
+        } else {
+            r.j1 = (t * a.antenna_count + x) * fsc;
+            r.j2 = (t * a.antenna_count + y) * fsc;
+        }
+    }
+    out[i] = r;
+}
+
+__global__ void k_chan_table(GridArgs a, uint32_t n, ChanRec *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ChanRec c;
+    c.rw = 1.0 / (double)a.wavelengths[i];
+    const uint64_t g = a.cgi[i];
+    c.cgi = g > 0xffffffffull ? 0xffffffffu : (uint32_t)g;
+    c.enabled = a.enabled[i] ? 1u : 0u;
+    out[i] = c;
+}
+
+hipError_t build_row_tables(const GridArgs &a, uint32_t rows, bool jones, RowRec *rowrec,
+                            ChanRec *chanrec, hipStream_t st) {
+    const uint32_t nfc = a.spw_count * a.chan;
+    if (nfc) k_chan_table<<<(nfc + 255) / 256, 256, 0, st>>>(a, nfc, chanrec);
+    if (rows) {
+        const uint64_t fsc = (uint64_t)a.facets * a.spw_count * a.chan;
+        k_row_table<<<(rows + 255) / 256, 256, 0, st>>>(a, rows, jones ? 1 : 0, fsc, rowrec);
+    }
+    return hipGetLastError();
+}
+
 // A kernel, not hipMemsetAsync, so that it can be captured into a graph and replayed.
 hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
     if (!n) return hipSuccess;

@@ -62,27 +62,75 @@ template <int MODE, int PATH, int W> struct GridderW {
         return v >= W ? v - W : v;
     }
 
-    // Prepare one sample; returns false when it contributes nothing.
+    // 16-byte vector loads of N consecutive values (the caller guarantees the alignment).
+    template <typename V, int N>
+    static __device__ __forceinline__ void load_vec(const V *p, V (&o)[N]) {
+        static_assert((N * sizeof(V)) % 16 == 0, "whole 16-byte words");
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const u4 *q = reinterpret_cast<const u4 *>(p);
+#pragma unroll
+        for (int i = 0; i < (int)(N * sizeof(V) / 16); ++i) {
+            const u4 x = q[i];
+            __builtin_memcpy(reinterpret_cast<char *>(o) + 16 * i, &x, 16);
+        }
+    }
+
+    // Prepare sample (sorted row ri, channel c); returns false when it contributes nothing.
+    // The checks and their order (and so the errors raised) are those of a gather straight from
+    // the row arrays: spw range, enabled channel, grid slice range, row flag and field, then the
+    // Jones ids.
     static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom *__restrict__ gp,
-                                                const T *sconv, uint32_t facet, uint32_t row,
-                                                uint32_t c, R_t &r, double nw[NC]) {
-        const uint32_t s = a.spw[row];
+                                                const T *sconv, const ChanRec *schan,
+                                                uint32_t ri, uint64_t jfacet, uint32_t c, R_t &r,
+                                                double nw[NC]) {
+        // the whole record in one round trip: the empty asm takes the four words as operands,
+        // so the compiler cannot sink the uvw and Jones-offset loads below the checks (which
+        // would make them a second dependent load)
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const u4 *rq = reinterpret_cast<const u4 *>(a.rowrec + ri);
+        u4 q0 = rq[0], q1 = rq[1], q2 = rq[2], q3 = rq[3];
+        asm volatile("" : "+v"(q0), "+v"(q1), "+v"(q2), "+v"(q3));
+        RowRec rr;
+        __builtin_memcpy(reinterpret_cast<char *>(&rr), &q0, 16);
+        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 16, &q1, 16);
+        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 32, &q2, 16);
+        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 48, &q3, 16);
+        const uint32_t s = rr.spw;
         if (s >= a.spw_count) {
             atomicOr(a.error_flag, 4);
             return false;
         }
         const uint32_t fc = s * a.chan + c;
-        if (!a.enabled[fc]) return false;
-        const uint64_t cgi = a.cgi[fc];
-        if (cgi >= a.cube) {
+        const ChanRec ch = a.chan_lds ? schan[fc] : a.chanrec[fc];
+        if (!ch.enabled) return false;
+        if (ch.cgi >= a.cube) {
             atomicOr(a.error_flag, 4);
             return false;
         }
-        if (a.row_flags[row] || a.field[row] != a.imaging_field) return false;
-        const double rw = 1.0 / (double)a.wavelengths[fc];
-        double u = (double)a.uvw[3 * (size_t)row] * rw;
-        double v = (double)a.uvw[3 * (size_t)row + 1] * rw;
-        const double w = (double)a.uvw[3 * (size_t)row + 2] * rw;
+        if (!(rr.state & ROW_USABLE)) return false;
+        if constexpr (MODE == MODE_4J) {
+            // raised here, before the visibility loads, so that the Jones loads issue with them
+            // (nothing between this point and the Jones correction can drop the sample)
+            if (rr.state & ROW_BAD_IDS) {
+                atomicOr(a.error_flag, 4);
+                return false;
+            }
+        }
+        const uint32_t row = rr.row;
+        const double rw = ch.rw;
+        double u = rr.u * rw;
+        double v = rr.v * rw;
+        const double w = rr.w * rw;
+        cpx<T> P[4], Q[4];
+        if constexpr (MODE == MODE_4J) {
+            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * (rr.j1 + jfacet + fc);
+            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * (rr.j2 + jfacet + fc);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                P[k] = Pi[k];
+                Q[k] = Qi[k];
+            }
+        }
         cpx<T> vis[NC];
         T wt[NC];
         if constexpr (MODE == MODE_PSF) {
@@ -90,31 +138,36 @@ template <int MODE, int PATH, int W> struct GridderW {
             wt[0] = T(1);
         } else {
             const size_t base = ((size_t)row * a.chan + c) * a.npol;
+            if constexpr (NC == 4) {
+                if (a.pol_contig) {
+                    visibility_base_type vv[8];
+                    visibility_weights_base_type ww[4];
+                    load_vec(a.vis + 2 * base, vv);
+                    load_vec(a.weights + base, ww);
+                    const uint32_t fl = *reinterpret_cast<const uint32_t *>(a.flags + base);
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                const size_t idx = base + a.pol_idx[k];
-                vis[k] = {(T)a.vis[2 * idx], (T)a.vis[2 * idx + 1]};
-                wt[k] = a.flags[idx] ? T(0) : (T)a.weights[idx];
+                    for (int k = 0; k < 4; ++k) {
+                        vis[k] = {(T)vv[2 * k], (T)vv[2 * k + 1]};
+                        wt[k] = ((fl >> (8 * k)) & 0xffu) ? T(0) : (T)ww[k];
+                    }
+                }
+            }
+            if (NC != 4 || !a.pol_contig) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    const size_t idx = base + a.pol_idx[k];
+                    vis[k] = {(T)a.vis[2 * idx], (T)a.vis[2 * idx + 1]};
+                    wt[k] = a.flags[idx] ? T(0) : (T)a.weights[idx];
+                }
             }
         }
         if constexpr (MODE == MODE_4J) {  // V = P^-1 V (Q^-1)^H (correlation_gridding_policies.h:523-542)
-            const size_t t = a.time_ids[row];
-            if (t >= a.ntime || a.ant1[row] >= a.antenna_count || a.ant2[row] >= a.antenna_count) {
-                atomicOr(a.error_flag, 4);
-                return false;
-            }
-            const size_t j1 = ((((size_t)t * a.antenna_count + a.ant1[row]) * a.facets + facet) *
-                                   a.spw_count + s) * a.chan + c;
-            const size_t j2 = ((((size_t)t * a.antenna_count + a.ant2[row]) * a.facets + facet) *
-                                   a.spw_count + s) * a.chan + c;
-            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j1;
-            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j2;
-            const cpx<T> h0 = conjg(Qi[0]), h1 = conjg(Qi[2]), h2 = conjg(Qi[1]), h3 = conjg(Qi[3]);
+            const cpx<T> h0 = conjg(Q[0]), h1 = conjg(Q[2]), h2 = conjg(Q[1]), h3 = conjg(Q[3]);
             const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
             const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));
             const cpx<T> t2 = cadd(cmul(vis[2], h0), cmul(vis[3], h2));
             const cpx<T> t3 = cadd(cmul(vis[2], h1), cmul(vis[3], h3));
-            const cpx<T> p0 = Pi[0], p1 = Pi[1], p2 = Pi[2], p3 = Pi[3];
+            const cpx<T> p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
             vis[0] = cadd(cmul(p0, t0), cmul(p1, t2));
             vis[1] = cadd(cmul(p0, t1), cmul(p1, t3));
             vis[2] = cadd(cmul(p2, t0), cmul(p3, t2));
@@ -161,7 +214,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         if (x0 < 0 || y0 < 0 || x0 + W + 2 >= (int)a.nx || y0 + W + 2 >= (int)a.ny) return false;
         r.x0 = x0;
         r.y0 = y0;
-        r.cgi = (int)cgi;
+        r.cgi = (int)ch.cgi;
         const int xm = x0 % W, ym = y0 % W;
 #pragma unroll
         for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
@@ -298,19 +351,21 @@ template <int MODE, int PATH, int W> struct GridderW {
 
     // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
     // early for lanes without a stream, run() then publishes for all of them.
-    static __device__ void run(const GridArgs &a, const T *sconv, unsigned char *swave) {
+    static __device__ void run(const GridArgs &a, const T *sconv, const ChanRec *schan,
+                               unsigned char *swave) {
         int nkey = -1;
         double nacc[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nacc[k] = 0;
         unsigned long long cnt = 0;
-        walk(a, sconv, swave, nkey, nacc, cnt);
+        walk(a, sconv, schan, swave, nkey, nacc, cnt);
         if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
         publish_counts(a, cnt, cnt * (unsigned long long)(W * W));
     }
 
     static __device__ __forceinline__ void walk(const GridArgs &a, const T *sconv,
-                                                unsigned char *swave, int &nkey,
+                                                const ChanRec *schan, unsigned char *swave,
+                                                int &nkey,
                                                 double (&nacc)[NC], unsigned long long &cnt) {
         const int lane = threadIdx.x & 63;
         const int gw = lane / G;
@@ -337,6 +392,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         const uint32_t nsamp = nrows * ncb;
         const bool tin = *a.time_inner != 0;
         const FacetGeom *gp = a.geom + facet;
+        const uint64_t jfacet = (uint64_t)facet * a.spw_count * a.chan;
 
         const int ua = gl % W;          // u residue of this lane
         const int c0 = (gl / W) * CPL;  // first correlation of this lane
@@ -362,9 +418,8 @@ template <int MODE, int PATH, int W> struct GridderW {
                     uint32_t tr, j;
                     stream_sample(s, nrows, ncb, tin, tr, j);
                     const uint32_t c = c_lo + j;
-                    const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
-                    if (prep(a, gp, sconv, facet, row, c, rec, nw)) {
+                    if (prep(a, gp, sconv, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
                         ++cnt;
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
@@ -479,8 +534,12 @@ __global__ __launch_bounds__(BLOCK) void romein_w_kernel(GridArgs a) {
     const uint32_t conv_bytes = (a.csz * sizeof(T) + 15) & ~15u;
     const T *gconv = reinterpret_cast<const T *>(a.conv);
     for (uint32_t i = threadIdx.x; i < a.csz; i += BLOCK) sconv[i] = gconv[i];
+    ChanRec *schan = reinterpret_cast<ChanRec *>(smem + conv_bytes);
+    const uint32_t nfc = a.chan_lds ? a.spw_count * a.chan : 0u;
+    for (uint32_t i = threadIdx.x; i < nfc; i += BLOCK) schan[i] = a.chanrec[i];
     __syncthreads();
-    Gd::run(a, sconv, smem + conv_bytes + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
+    Gd::run(a, sconv, schan,
+            smem + conv_bytes + nfc * sizeof(ChanRec) + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
 template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &a, hipStream_t st) {
@@ -489,7 +548,9 @@ template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &
     const uint64_t blocks = (waves + BLOCK / 64 - 1) / (BLOCK / 64);
     if (blocks == 0) return hipSuccess;
     const size_t smem = ((a.csz * sizeof(typename Gd::T) + 15) & ~15u) +
+                        (a.chan_lds ? (size_t)a.spw_count * a.chan * sizeof(ChanRec) : 0) +
                         (BLOCK / 64) * (size_t)Gd::WAVE_BYTES;
+    if (!a.rowrec || !a.chanrec) return hipErrorInvalidValue;
     romein_w_kernel<MODE, PATH, W><<<dim3((uint32_t)blocks), dim3(BLOCK), smem, st>>>(a);
     return hipGetLastError();
 }

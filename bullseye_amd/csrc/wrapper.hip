@@ -41,7 +41,7 @@ struct State {
     int path = PATH_FP64;
     // device buffers
     DevBuf grid, psfgrid, norm, counters, errflag, conv_f, conv_d, tapsum, geom, taper;
-    DevBuf perm, blstart, keys, vals, keys_out, counts, ws;
+    DevBuf perm, blstart, keys, vals, keys_out, counts, ws, rowrec, chanrec;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
         in_jones, in_wl, in_enabled, in_cgi;
     DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp, dft_smp,
@@ -516,8 +516,25 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
         }
         HIP_TRY(hipStreamSynchronize(g.stream));  // wlist_host stays valid until here
     } else {
+        const bool specialised = romein_w_groups_per_wave(W, mode, g.path) > 0 &&
+                                 !g.opts.reserved[0];
+        if (specialised) {
+            // sorted row and channel tables for the specialised kernel's sample preparation
+            const size_t nfc = nspw * C;
+            HIP_TRY(g.rowrec.ensure(std::max<size_t>(1, rows) * sizeof(RowRec)));
+            HIP_TRY(g.chanrec.ensure(std::max<size_t>(1, nfc) * sizeof(ChanRec)));
+            a.rowrec = g.rowrec.as<RowRec>();
+            a.chanrec = g.chanrec.as<ChanRec>();
+            a.chan_lds = nfc <= CHAN_LDS_MAX ? 1u : 0u;
+            const bool four = mode == MODE_4 || mode == MODE_4J;
+            a.pol_contig = (four && !psf && p->number_of_polarization_terms == 4 &&
+                            ((uintptr_t)a.vis % 16) == 0 && ((uintptr_t)a.weights % 16) == 0 &&
+                            ((uintptr_t)a.flags % 4) == 0) ? 1u : 0u;
+            HIP_TRY(build_row_tables(a, (uint32_t)rows, mode == MODE_4J && rows > 0,
+                                     g.rowrec.as<RowRec>(), g.chanrec.as<ChanRec>(), g.stream));
+        }
         HIP_TRY(hipEventRecord(tc.k0, g.stream));
-        if (romein_w_groups_per_wave(W, mode, g.path) > 0 && !g.opts.reserved[0])
+        if (specialised)
             HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
         else
             HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
@@ -815,6 +832,7 @@ int releaseLibrary(void) {
     DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv_f,
                       &g.conv_d, &g.tapsum,
                       &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
+                      &g.rowrec, &g.chanrec,
                       &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
