@@ -218,10 +218,16 @@ template <int MODE, int PATH, int W> struct GridderW {
         const int xm = x0 % W, ym = y0 % W;
 #pragma unroll
         for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
+        // tap for residue j: sconv[f + mod_w(j - m) * OS] = sconv[f - m*OS + j*OS (+ W*OS if j < m)],
+        // indexes formed by additions (no per-tap multiply)
+        const int os = (int)a.OS, wos = W * os;
+        const int bu = fu - xm * os, bv = fv - ym * os;
+        int jos = 0;
 #pragma unroll
         for (int j = 0; j < W; ++j) {
-            r.cu[j] = sconv[fu + mod_w(j - xm) * (int)a.OS];
-            r.cv[j] = sconv[fv + mod_w(j - ym) * (int)a.OS];
+            r.cu[j] = sconv[bu + jos + (j < xm ? wos : 0)];
+            r.cv[j] = sconv[bv + jos + (j < ym ? wos : 0)];
+            jos += os;
         }
         const double ts = a.tapsum[fu] * a.tapsum[fv];
 #pragma unroll
@@ -231,15 +237,12 @@ template <int MODE, int PATH, int W> struct GridderW {
 
     typedef acc1<T, KAHAN> Acc[W][CPL][2];
 
-    static __device__ __forceinline__ void flush1(const GridArgs &a, uint32_t facet, int cgi,
-                                                  int x, int y, int c0,
-                                                  acc1<T, KAHAN> (&acc)[CPL][2]) {
+    // Adds one grid point's sums into the FP64 grid (skipped when they are zero) and clears them.
+    static __device__ __forceinline__ void flush1(double *cell, acc1<T, KAHAN> (&acc)[CPL][2]) {
         bool nz = false;
 #pragma unroll
         for (int k = 0; k < CPL; ++k) nz |= (acc[k][0].s != T(0)) | (acc[k][1].s != T(0));
         if (nz) {
-            double *cell = a.grid + (((((size_t)facet * a.cube + cgi) * a.ny + y) * a.nx + x) *
-                                         (size_t)a.pstride + c0) * 2;
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
                 unsafeAtomicAdd(cell + 2 * k, acc[k][0].value());
@@ -253,21 +256,34 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
     }
 
-    // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.
+    // Grid address of (row 0, column xo, correlation c0) of slice (facet, cgi); row y is
+    // y * nx * pstride * 2 doubles further.
+    static __device__ __forceinline__ double *column(const GridArgs &a, uint32_t facet, int cgi,
+                                                     int xo, int c0) {
+        return a.grid + ((((size_t)facet * a.cube + cgi) * a.ny * a.nx + xo) * (size_t)a.pstride +
+                         c0) * 2;
+    }
+
+    // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.  The
+    // lane owns, for v-residue r, the row of the footprint's window [py0, py0 + W) congruent to
+    // r mod W, and the column of [px0, px0 + W) congruent to its u-residue; a point stays owned
+    // exactly when it lies inside the new window too.
     static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int ua,
                                                    int c0, int x0, int y0, int cgi, int &px0,
                                                    int &py0, int &pcgi, Acc &acc) {
         if (x0 == px0 && y0 == py0 && cgi == pcgi) return;
         if (pcgi >= 0) {
             const int xo = px0 + mod_w(ua - px0 % W);
-            const int xn = x0 + mod_w(ua - x0 % W);
-            const bool all = (cgi != pcgi) || (xn != xo);
-            const int pym = py0 % W, ym = y0 % W;
+            const bool all = (cgi != pcgi) || ((unsigned)(xo - x0) >= (unsigned)W);
+            const int pym = py0 % W;
+            const int ybase = py0 - pym;
+            double *col = column(a, facet, pcgi, xo, c0);
+            const uint32_t rs = a.nx * a.pstride * 2u;   // < 2^32: nx is limited at init
 #pragma unroll
             for (int r = 0; r < W; ++r) {
-                const int yo = py0 + mod_w(r - pym);
-                const int yn = y0 + mod_w(r - ym);
-                if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+                const int yo = ybase + r + (r < pym ? W : 0);   // >= 0: footprints lie in the grid
+                if (all || (unsigned)(yo - y0) >= (unsigned)W)
+                    flush1(col + (size_t)(uint32_t)yo * rs, acc[r]);
             }
         }
         px0 = x0;
@@ -518,8 +534,11 @@ template <int MODE, int PATH, int W> struct GridderW {
         if (pcgi >= 0) {
             const int xo = px0 + mod_w(ua - px0 % W);
             const int pym = py0 % W;
+            double *col = column(a, facet, pcgi, xo, c0);
+            const uint32_t rs = a.nx * a.pstride * 2u;
 #pragma unroll
-            for (int r = 0; r < W; ++r) flush1(a, facet, pcgi, xo, py0 + mod_w(r - pym), c0, acc[r]);
+            for (int r = 0; r < W; ++r)
+                flush1(col + (size_t)(uint32_t)(py0 + mod_w(r - pym)) * rs, acc[r]);
         }
         nkey = ncgi >= 0 ? (int)(facet * a.cube + ncgi) : -1;
     }

@@ -725,6 +725,8 @@ int initLibrary(gridding_parameters *p) {
                                  : PATH_F32K;
 #endif
     if (g.nx == 0 || g.ny == 0 || g.OS == 0) return fail("nx, ny and conv_oversample must be > 0");
+    // kernels form grid row strides (nx * 4 correlations * 2) in 32 bits
+    if (g.nx > (1u << 28) || g.ny > (1u << 28)) return fail("nx and ny must be <= 2^28");
     if (2 * g.S + 3 >= g.nx || 2 * g.S + 3 >= g.ny) return fail("conv_support too large for the grid");
     if (!p->conv) return fail("conv is NULL");
     HIP_TRY(g.grid.ensure(grid_elems() * sizeof(double)));
