@@ -9,11 +9,12 @@
 //    paths, 2 on the FP64 path), which cuts the accumulator registers per lane;
 //  * FP32 compensated sums take one Kahan step per run of samples on an unchanged footprint:
 //    the run's products are chained by FMA onto -c and the result is added with the Kahan
-//    correction.  A run is made of uniform quads (QUAD consecutive samples whose valid members
-//    share one footprint), at most one step of G samples long; the leader lane of each quad
-//    decides uniformity once per step, so the walking lanes read one flag per quad.  Mixed
-//    quads take one Kahan step per sample.  The error stays O(u) per term times the run
-//    length, independent of the number of samples a register sums.
+//    correction.  A run is at most one step of G samples long.  The leader lane of each quad
+//    (QUAD consecutive samples) decides once per step whether the quad's valid samples share
+//    one footprint, so the walking lanes read one flag per uniform quad; in a mixed quad each
+//    sample either extends the open run (same footprint) or closes it and opens one on its
+//    own footprint.  The error stays O(u) per term times the run length, independent of the
+//    number of samples a register sums.
 // A lane group is W * NSPLIT lanes: lane (h, a) owns u-residue a and correlations
 // [h*CPL, (h+1)*CPL) of all W v-residues.  Each group's LDS records start on a different
 // LDS bank so the four groups of a wavefront read without conflicts.
@@ -80,8 +81,9 @@ template <int MODE, int PATH, int W> struct GridderW {
     // the row arrays: spw range, enabled channel, grid slice range, row flag and field, then the
     // Jones ids.
     static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom *__restrict__ gp,
-                                                const T *sconv, const ChanRec *schan,
-                                                uint32_t ri, uint64_t jfacet, uint32_t c, R_t &r,
+                                                const T *sconv, const double *sts,
+                                                const ChanRec *schan, uint32_t ri,
+                                                uint64_t jfacet, uint32_t c, R_t &r,
                                                 double nw[NC]) {
         // the whole record in one round trip: the empty asm takes the four words as operands,
         // so the compiler cannot sink the uvw and Jones-offset loads below the checks (which
@@ -229,7 +231,7 @@ template <int MODE, int PATH, int W> struct GridderW {
             r.cv[j] = sconv[bv + jos + (j < ym ? wos : 0)];
             jos += os;
         }
-        const double ts = a.tapsum[fu] * a.tapsum[fv];
+        const double ts = sts[fu] * sts[fv];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nw[k] = (double)wt[k] * ts;
         return true;
@@ -340,47 +342,50 @@ template <int MODE, int PATH, int W> struct GridderW {
             }
     }
 
-    // QUAD samples on the run's footprint: products chained by FMA onto the run.
-    static __device__ __forceinline__ void run_quad(const R_t *recs, int ua, int c0, Run &y) {
+    // One sample on the run's footprint: its products chained by FMA onto the run.
+    static __device__ __forceinline__ void run_one(const R_t &rec, int ua, int c0, Run &y) {
+        const T cu = rec.cu[ua];
+        cpx<T> vs[CPL];
 #pragma unroll
-        for (int q = 0; q < QUAD; ++q) {
-            const R_t &rec = recs[q];
-            const T cu = rec.cu[ua];
-            cpx<T> vs[CPL];
+        for (int k = 0; k < CPL; ++k) {
+            vs[k] = rec.vis[c0 + k];
+            vs[k].re *= cu;
+            vs[k].im *= cu;
+        }
+#pragma unroll
+        for (int r = 0; r < W; ++r) {
+            const T cv = rec.cv[r];
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
-                vs[k] = rec.vis[c0 + k];
-                vs[k].re *= cu;
-                vs[k].im *= cu;
-            }
-#pragma unroll
-            for (int r = 0; r < W; ++r) {
-                const T cv = rec.cv[r];
-#pragma unroll
-                for (int k = 0; k < CPL; ++k) {
-                    y[r][k][0] = fma(vs[k].re, cv, y[r][k][0]);
-                    y[r][k][1] = fma(vs[k].im, cv, y[r][k][1]);
-                }
+                y[r][k][0] = fma(vs[k].re, cv, y[r][k][0]);
+                y[r][k][1] = fma(vs[k].im, cv, y[r][k][1]);
             }
         }
     }
 
+    // QUAD samples on the run's footprint.
+    static __device__ __forceinline__ void run_quad(const R_t *recs, int ua, int c0, Run &y) {
+#pragma unroll
+        for (int q = 0; q < QUAD; ++q) run_one(recs[q], ua, c0, y);
+    }
+
     // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
     // early for lanes without a stream, run() then publishes for all of them.
-    static __device__ void run(const GridArgs &a, const T *sconv, const ChanRec *schan,
-                               unsigned char *swave) {
+    static __device__ void run(const GridArgs &a, const T *sconv, const double *sts,
+                               const ChanRec *schan, unsigned char *swave) {
         int nkey = -1;
         double nacc[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nacc[k] = 0;
         unsigned long long cnt = 0;
-        walk(a, sconv, schan, swave, nkey, nacc, cnt);
+        walk(a, sconv, sts, schan, swave, nkey, nacc, cnt);
         if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
         publish_counts(a, cnt, cnt * (unsigned long long)(W * W));
     }
 
     static __device__ __forceinline__ void walk(const GridArgs &a, const T *sconv,
-                                                const ChanRec *schan, unsigned char *swave,
+                                                const double *sts, const ChanRec *schan,
+                                                unsigned char *swave,
                                                 int &nkey,
                                                 double (&nacc)[NC], unsigned long long &cnt) {
         const int lane = threadIdx.x & 63;
@@ -435,7 +440,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                     stream_sample(s, nrows, ncb, tin, tr, j);
                     const uint32_t c = c_lo + j;
                     double nw[NC];
-                    if (prep(a, gp, sconv, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
+                    if (prep(a, gp, sconv, sts, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
                         ++cnt;
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
@@ -507,16 +512,20 @@ template <int MODE, int PATH, int W> struct GridderW {
                         }
                         run_quad(grp + q * QUAD, ua, c0, y);
                     } else {
-                        if (open) {
-                            close_run(acc, y);
-                            open = false;
-                        }
+                        // a mixed quad: a sample on the open run's footprint extends the run;
+                        // one elsewhere closes it, moves the ownership and opens a new run
 #pragma unroll
                         for (int k2 = 0; k2 < QUAD; ++k2) {
                             const R_t &rr = grp[q * QUAD + k2];
                             if (rr.cgi < 0) continue;
-                            move_to(a, facet, ua, c0, rr.x0, rr.y0, rr.cgi, px0, py0, pcgi, acc);
-                            add_one(rr, ua, c0, acc);
+                            if (!open || rr.x0 != px0 || rr.y0 != py0 || rr.cgi != pcgi) {
+                                if (open) close_run(acc, y);
+                                move_to(a, facet, ua, c0, rr.x0, rr.y0, rr.cgi, px0, py0, pcgi,
+                                        acc);
+                                open_run(acc, y);
+                                open = true;
+                            }
+                            run_one(rr, ua, c0, y);
                         }
                     }
                 }
@@ -553,12 +562,17 @@ __global__ __launch_bounds__(BLOCK) void romein_w_kernel(GridArgs a) {
     const uint32_t conv_bytes = (a.csz * sizeof(T) + 15) & ~15u;
     const T *gconv = reinterpret_cast<const T *>(a.conv);
     for (uint32_t i = threadIdx.x; i < a.csz; i += BLOCK) sconv[i] = gconv[i];
-    ChanRec *schan = reinterpret_cast<ChanRec *>(smem + conv_bytes);
+    // tap sums per fraction (OS + 1 doubles), then the channel table when it is staged
+    double *sts = reinterpret_cast<double *>(smem + conv_bytes);
+    const uint32_t ts_bytes = ((a.OS + 1) * 8 + 15) & ~15u;
+    for (uint32_t i = threadIdx.x; i <= a.OS; i += BLOCK) sts[i] = a.tapsum[i];
+    ChanRec *schan = reinterpret_cast<ChanRec *>(smem + conv_bytes + ts_bytes);
     const uint32_t nfc = a.chan_lds ? a.spw_count * a.chan : 0u;
     for (uint32_t i = threadIdx.x; i < nfc; i += BLOCK) schan[i] = a.chanrec[i];
     __syncthreads();
-    Gd::run(a, sconv, schan,
-            smem + conv_bytes + nfc * sizeof(ChanRec) + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
+    Gd::run(a, sconv, sts, schan,
+            smem + conv_bytes + ts_bytes + nfc * sizeof(ChanRec) +
+                (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
 template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &a, hipStream_t st) {
@@ -567,6 +581,7 @@ template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &
     const uint64_t blocks = (waves + BLOCK / 64 - 1) / (BLOCK / 64);
     if (blocks == 0) return hipSuccess;
     const size_t smem = ((a.csz * sizeof(typename Gd::T) + 15) & ~15u) +
+                        (((a.OS + 1) * 8 + 15) & ~15u) +
                         (a.chan_lds ? (size_t)a.spw_count * a.chan * sizeof(ChanRec) : 0) +
                         (BLOCK / 64) * (size_t)Gd::WAVE_BYTES;
     if (!a.rowrec || !a.chanrec) return hipErrorInvalidValue;
