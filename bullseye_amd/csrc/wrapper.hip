@@ -230,7 +230,7 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
                            int mode, bool same_slice, uint32_t &cps, uint32_t &nseg) {
     const size_t C = p->channel_count;
     cps = g.opts.channels_per_stream ? g.opts.channels_per_stream
-                                     : (uint32_t)(same_slice ? std::min<size_t>(C, 16) : 1);
+                                     : (uint32_t)(same_slice ? std::min<size_t>(C, 32) : 1);
     cps = (uint32_t)std::max<size_t>(1, std::min<size_t>(cps, C));
     const size_t nblocks = (C + cps - 1) / cps;
     if (g.opts.rows_per_stream) {
