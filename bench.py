@@ -83,18 +83,44 @@ def calibration_copy(mib):
     del a, b
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch of the workload's gridding kernel(s): rocprofv3 FETCH_SIZE plus
-    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB device copy, as summarised in
-    the committed profile `profiles/<name>`; None when the summary is absent."""
+def profiled_value(name, key):
+    """`key = value` from the committed profile summary `profiles/<name>`; None when absent."""
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
+                if line.startswith(key + " ="):
                     return float(line.split("=")[1].split()[0])
     except OSError:
         return None
     return None
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of the workload's gridding kernel(s): rocprofv3 FETCH_SIZE plus
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB device copy, as summarised in
+    the committed profile `profiles/<name>`; None when the summary is absent."""
+    return profiled_value(name, "traffic_bytes_per_launch")
+
+
+def flush_atomics(lib, profile, kernel_ms):
+    """The grid flushes against the device's FP64 atomic-add rate: atomics per launch
+    (rocprofv3 TCC_ATOMIC in the committed profile), their rate over the measured kernel time,
+    and the rate bullseye_fp64_atomic_rate measures now (every flush is such an add, executed
+    on the memory side)."""
+    rate = ctypes.c_double(0.0)
+    ceiling = None
+    if lib.bullseye_fp64_atomic_rate(ctypes.byref(rate)) == 0:
+        ceiling = rate.value / 1e9
+    else:
+        log("atomic rate: %s" % lib.bullseye_last_error().decode())
+    per = profiled_value(profile, "atomics_per_launch")
+    out = {"per_launch": per, "gops": None, "ceiling_gops": None if ceiling is None
+           else round(ceiling, 3), "frac": None}
+    if per is not None and kernel_ms > 0:
+        out["gops"] = round(per / (kernel_ms * 1e-3) / 1e9, 3)
+        if ceiling:
+            out["frac"] = round(out["gops"] / ceiling, 4)
+    return out
 
 
 def rel_errors(got, want):
@@ -233,6 +259,8 @@ def main():
                      "traffic": profiled_traffic("romein_w_facet_4_cor_corrections_fp32comp_pmc.md"),
                      "kernel_ms": round(k32, 3),
                      "flops_per_launch": flops},
+        "flush_atomics": flush_atomics(
+            lib, "romein_w_facet_4_cor_corrections_fp32comp_pmc.md", k32),
     }
 
     if not args.no_cpu:
@@ -368,6 +396,7 @@ def main_wproj(args):
                      "frac": round(flops / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                      "traffic": profiled_traffic("wproj_grid_single_pol_fp32comp_pmc.md"),
                      "kernel_ms": round(k32, 3), "flops_per_launch": flops},
+        "flush_atomics": flush_atomics(lib, "wproj_grid_single_pol_fp32comp_pmc.md", k32),
     }
     if not args.no_cpu:
         from oracle import oracle
@@ -683,6 +712,7 @@ def main_batch(args):
                      "frac": round(flops / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                      "traffic": profiled_traffic("batch_facets_fp32comp_pmc.md"),
                      "kernel_ms": round(k32, 3), "flops_per_launch": flops},
+        "flush_atomics": flush_atomics(lib, "batch_facets_fp32comp_pmc.md", k32),
     }
     if tune is not None:
         result["autotune"] = {"timesteps": args.tune_timesteps,

@@ -15,7 +15,7 @@ from . import LIB_DIR, LIBRARY_NAMES, PACKAGE_DIR, PRECISIONS
 
 CSRC = os.path.join(PACKAGE_DIR, "csrc")
 SOURCES = ("romein.hip", "romein_w.hip", "wproj.hip", "jones.hip", "ingest.hip",
-           "finalize.hip", "degrid.hip", "dft.hip", "batch.hip", "wrapper.hip")
+           "finalize.hip", "degrid.hip", "dft.hip", "batch.hip", "calibrate.hip", "wrapper.hip")
 DEFINES = {"single": "BULLSEYE_SINGLE", "double": "BULLSEYE_DOUBLE", "mixed": "BULLSEYE_MIXED"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -25,7 +25,7 @@ EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_ge
                      "bullseye_dft_predict", "bullseye_batch_kernel_set",
                      "bullseye_batch_set_tuning", "bullseye_batch_get_tuning",
                      "bullseye_batch_plan", "bullseye_batch_execute", "bullseye_batch_autotune",
-                     "bullseye_batch_release")
+                     "bullseye_batch_release", "bullseye_fp64_atomic_rate")
 
 
 class bullseye_options(ctypes.Structure):
@@ -123,6 +123,7 @@ def load_library(architecture, precision, share_torch_runtime=True):
     lib.bullseye_set_options.argtypes = [ctypes.POINTER(bullseye_options)]
     lib.bullseye_get_options.argtypes = [ctypes.POINTER(bullseye_options)]
     lib.bullseye_get_stats.argtypes = [ctypes.POINTER(bullseye_stats)]
+    lib.bullseye_fp64_atomic_rate.argtypes = [ctypes.POINTER(ctypes.c_double)]
     lib.bullseye_degrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p]
     lib.bullseye_dft_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p]

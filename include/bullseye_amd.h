@@ -230,6 +230,13 @@ typedef struct bullseye_stats {
 } bullseye_stats;
 int bullseye_get_stats(bullseye_stats *stats);
 
+/* Measured rate of FP64 atomic adds to device memory on the current device, in lane-operations
+ * per second: 2^27 adds of 1.0 at hashed addresses of a 16 MB buffer after one warm-up launch.
+ * Every grid flush is such an add, and they execute on the memory side, so this is the ceiling
+ * of a flush-bound gridder.  No counterpart in the reference (a measurement for the
+ * benchmark's roofline).  Needs no initLibrary; allocates and frees its own buffer. */
+int bullseye_fp64_atomic_rate(double *ops_per_second);
+
 /* Gridding modes, numbered in the order of the entry points above. */
 enum bullseye_grid_mode {
     BULLSEYE_GRID_SINGLE_POL = 0,
