@@ -73,7 +73,7 @@ struct Plan {
     bool ready = false;
     int mode = MODE_1, schedule = 0, path = PATH_F32K;
     uint32_t n = 0, NC = 1, cube = 1, sfC = 1, cubeB = 1;
-    uint32_t B = 0, C = 0, cps = 1, ncb = 1, base_rows = 32, nseg_base = 1;
+    uint32_t B = 0, C = 0, cps = 1, ncb = 1, base_rows = 32, nseg_base = 1, rows = 0;
     uint64_t nbase = 0;
     std::vector<bullseye_batch_facet> facets;
     std::vector<int> fkind;                  // MODE_x of each facet
@@ -84,7 +84,7 @@ struct Plan {
     DevBuf acc, norm, fargs, geom, lists, queues, cls_all, out_tmp, walk;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_wl,
         in_enabled, in_cgi, in_sfcgi;
-    DevBuf perm, keys, vals, keys_out, counts, blstart, ws;
+    DevBuf perm, keys, vals, keys_out, counts, blstart, ws, rowrec;
     std::vector<Launch> launches;
     std::vector<wp::WArgs> fargs_host;
     std::vector<FacetGeom> geom_host;
@@ -107,7 +107,7 @@ struct Plan {
         drop_graph();
         DevBuf *bufs[] = {&acc, &norm, &fargs, &geom, &lists, &queues, &cls_all, &out_tmp, &walk,
                           &in_vis, &in_uvw, &in_w, &in_flags, &in_rowflags, &in_field, &in_spw,
-                          &in_a1, &in_a2, &in_wl, &in_enabled, &in_cgi, &in_sfcgi, &perm, &keys,
+                          &in_a1, &in_a2, &in_wl, &in_enabled, &in_cgi, &in_sfcgi, &perm, &rowrec, &keys,
                           &vals, &keys_out, &counts, &blstart, &ws};
         for (DevBuf *b : bufs) b->release();
         for (int b = 0; b < BRANCHES - 1; ++b) {
@@ -411,6 +411,9 @@ hipError_t issue(const Plan &pl, const Launch &L, int shape, hipStream_t st) {
 
 // Clears and launches of one execution: on the library stream, branches forked off it.
 int enqueue_all(Plan &pl, hipStream_t st) {
+    // the row table is gathered from the row arrays on every pass: with device-resident buffers
+    // the caller's arrays are read in place, so changes after planning are seen
+    HIP_TRY(build_row_tables(pl.base, pl.rows, false, pl.rowrec.as<RowRec>(), nullptr, st));
     HIP_TRY(fill_f64(pl.acc.as<double>(), pl.acc_elems, 0.0, st));
     HIP_TRY(fill_f64(pl.norm.as<double>(), pl.norm_stride * pl.base.nslots, 0.0, st));
     if (pl.schedule != BULLSEYE_BATCH_PER_FACET)
@@ -585,6 +588,12 @@ int build_plan(gridding_parameters *p, int mode, const bullseye_batch_facet *fac
     HIP_TRY(pl.walk.ensure(sizeof(int)));
     a.time_inner = pl.walk.as<int>();
     HIP_TRY(launch_walk_order(a, lv.walk_order, pl.walk.as<int>(), lv.stream));
+    // sorted row table for the class kernels' sample preparation (rebuilt by every pass, see
+    // enqueue_all; this copy serves the autotuner's single launches)
+    pl.rows = (uint32_t)rows;
+    HIP_TRY(pl.rowrec.ensure(std::max<size_t>(1, rows) * sizeof(RowRec)));
+    a.rowrec = pl.rowrec.as<RowRec>();
+    HIP_TRY(build_row_tables(a, pl.rows, false, pl.rowrec.as<RowRec>(), nullptr, lv.stream));
 
     // per-facet accumulation grids, normalisation slots and geometry
     pl.fkind.resize(n);

@@ -57,6 +57,22 @@ struct __align__(16) ChanRec {
     uint32_t enabled;
 };
 static_assert(sizeof(RowRec) == 64 && sizeof(ChanRec) == 16, "table record sizes");
+
+// The whole row record in one round trip: the empty asm takes the four 16-byte words as
+// operands, so the compiler cannot sink the uvw and Jones-offset loads below the caller's
+// checks (which would make them a second dependent load).
+__device__ __forceinline__ RowRec load_row_record(const RowRec *p) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const u4 *q = reinterpret_cast<const u4 *>(p);
+    u4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    asm volatile("" : "+v"(q0), "+v"(q1), "+v"(q2), "+v"(q3));
+    RowRec r;
+    __builtin_memcpy(reinterpret_cast<char *>(&r), &q0, 16);
+    __builtin_memcpy(reinterpret_cast<char *>(&r) + 16, &q1, 16);
+    __builtin_memcpy(reinterpret_cast<char *>(&r) + 32, &q2, 16);
+    __builtin_memcpy(reinterpret_cast<char *>(&r) + 48, &q3, 16);
+    return r;
+}
 constexpr uint32_t CHAN_LDS_MAX = 1024;  // channel tables up to this size are staged in LDS
 
 // One Romein stream: a lane group walks the rows [row_lo, row_hi) of baseline `baseline`

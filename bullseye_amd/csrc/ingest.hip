@@ -195,7 +195,7 @@ __global__ void k_chan_table(GridArgs a, uint32_t n, ChanRec *out) {
 hipError_t build_row_tables(const GridArgs &a, uint32_t rows, bool jones, RowRec *rowrec,
                             ChanRec *chanrec, hipStream_t st) {
     const uint32_t nfc = a.spw_count * a.chan;
-    if (nfc) k_chan_table<<<(nfc + 255) / 256, 256, 0, st>>>(a, nfc, chanrec);
+    if (nfc && chanrec) k_chan_table<<<(nfc + 255) / 256, 256, 0, st>>>(a, nfc, chanrec);
     if (rows) {
         const uint64_t fsc = (uint64_t)a.facets * a.spw_count * a.chan;
         k_row_table<<<(rows + 255) / 256, 256, 0, st>>>(a, rows, jones ? 1 : 0, fsc, rowrec);

@@ -85,18 +85,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                                                 const ChanRec *schan, uint32_t ri,
                                                 uint64_t jfacet, uint32_t c, R_t &r,
                                                 double nw[NC]) {
-        // the whole record in one round trip: the empty asm takes the four words as operands,
-        // so the compiler cannot sink the uvw and Jones-offset loads below the checks (which
-        // would make them a second dependent load)
-        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-        const u4 *rq = reinterpret_cast<const u4 *>(a.rowrec + ri);
-        u4 q0 = rq[0], q1 = rq[1], q2 = rq[2], q3 = rq[3];
-        asm volatile("" : "+v"(q0), "+v"(q1), "+v"(q2), "+v"(q3));
-        RowRec rr;
-        __builtin_memcpy(reinterpret_cast<char *>(&rr), &q0, 16);
-        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 16, &q1, 16);
-        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 32, &q2, 16);
-        __builtin_memcpy(reinterpret_cast<char *>(&rr) + 48, &q3, 16);
+        const RowRec rr = load_row_record(a.rowrec + ri);
         const uint32_t s = rr.spw;
         if (s >= a.spw_count) {
             atomicOr(a.error_flag, 4);

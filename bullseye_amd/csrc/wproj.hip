@@ -76,10 +76,14 @@ template <int MODE, int PATH> struct WPrep {
     typedef typename PathTraits<PATH>::acc_t T;
     typedef WRec<T, NC> R_t;
 
+    // Sample (sorted row ri, channel c).  The checks and their order (and so the errors raised)
+    // are those of a gather straight from the row arrays: spw range, enabled channel, grid
+    // slice range, row flag and field, then the Jones ids.
     static __device__ __forceinline__ bool prep(const WArgs &w, int M, uint32_t facet,
-                                                uint32_t row, uint32_t c, R_t &r, double nw[NC]) {
+                                                uint32_t ri, uint32_t c, R_t &r, double nw[NC]) {
         const GridArgs &a = w.g;
-        const uint32_t s = a.spw[row];
+        const RowRec rr = load_row_record(a.rowrec + ri);
+        const uint32_t s = rr.spw;
         if (s >= a.spw_count) {
             atomicOr(a.error_flag, 4);
             return false;
@@ -91,11 +95,12 @@ template <int MODE, int PATH> struct WPrep {
             atomicOr(a.error_flag, 4);
             return false;
         }
-        if (a.row_flags[row] || a.field[row] != a.imaging_field) return false;
+        if (!(rr.state & ROW_USABLE)) return false;
+        const uint32_t row = rr.row;
         const double rw = 1.0 / (double)a.wavelengths[fc];
-        double u = (double)a.uvw[3 * (size_t)row] * rw;
-        double v = (double)a.uvw[3 * (size_t)row + 1] * rw;
-        double ww = (double)a.uvw[3 * (size_t)row + 2] * rw;
+        double u = rr.u * rw;
+        double v = rr.v * rw;
+        double ww = rr.w * rw;
         cpx<T> vis[NC];
         T wt[NC];
         if constexpr (MODE == MODE_PSF) {
@@ -111,17 +116,13 @@ template <int MODE, int PATH> struct WPrep {
             }
         }
         if constexpr (MODE == MODE_4J) {
-            const size_t t = a.time_ids[row];
-            if (t >= a.ntime || a.ant1[row] >= a.antenna_count || a.ant2[row] >= a.antenna_count) {
+            if (rr.state & ROW_BAD_IDS) {
                 atomicOr(a.error_flag, 4);
                 return false;
             }
-            const size_t j1 = ((((size_t)t * a.antenna_count + a.ant1[row]) * a.facets + facet) *
-                                   a.spw_count + s) * a.chan + c;
-            const size_t j2 = ((((size_t)t * a.antenna_count + a.ant2[row]) * a.facets + facet) *
-                                   a.spw_count + s) * a.chan + c;
-            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j1;
-            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * j2;
+            const uint64_t jf = ((uint64_t)facet * a.spw_count + s) * a.chan + c;
+            const cpx<T> *Pi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * (rr.j1 + jf);
+            const cpx<T> *Qi = reinterpret_cast<const cpx<T> *>(a.jones_inv) + 4 * (rr.j2 + jf);
             const cpx<T> h0 = conjg(Qi[0]), h1 = conjg(Qi[2]), h2 = conjg(Qi[1]), h3 = conjg(Qi[3]);
             const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
             const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));
@@ -346,11 +347,10 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                     uint32_t tr, j;
                     stream_sample(s, nrows, ncb, tin, tr, j);
                     const uint32_t c = c_lo + j;
-                    const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
                     const bool mine =
                         !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
-                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
+                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, b0 + r_lo + tr, c, rec, nw)) {
                         ++cnt;
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -657,11 +657,10 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
                     uint32_t tr, j;
                     stream_sample(s, nrows, ncb, tin, tr, j);
                     const uint32_t c = c_lo + j;
-                    const uint32_t row = a.perm[b0 + r_lo + tr];
                     double nw[NC];
                     const bool mine =
                         !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
-                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, row, c, rec, nw)) {
+                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, b0 + r_lo + tr, c, rec, nw)) {
                         ++cnt;
                         ntaps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {

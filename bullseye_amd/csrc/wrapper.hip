@@ -468,6 +468,18 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.time_inner = g.walk.as<int>();
     HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, g.walk.as<int>(), g.stream));
 
+    // sorted row table (every gridder's sample preparation) and channel table (the AA gridder)
+    {
+        const size_t nfc = nspw * C;
+        HIP_TRY(g.rowrec.ensure(std::max<size_t>(1, rows) * sizeof(RowRec)));
+        HIP_TRY(g.chanrec.ensure(std::max<size_t>(1, nfc) * sizeof(ChanRec)));
+        a.rowrec = g.rowrec.as<RowRec>();
+        a.chanrec = g.chanrec.as<ChanRec>();
+        a.chan_lds = nfc <= CHAN_LDS_MAX ? 1u : 0u;
+        HIP_TRY(build_row_tables(a, (uint32_t)rows, mode == MODE_4J && rows > 0,
+                                 g.rowrec.as<RowRec>(), g.chanrec.as<ChanRec>(), g.stream));
+    }
+
     if (wproj) {
         // w-projection: streams are SEG-row segments bucketed by their support class
         const uint32_t SEG = 32;
@@ -519,19 +531,10 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
         const bool specialised = romein_w_groups_per_wave(W, mode, g.path) > 0 &&
                                  !g.opts.reserved[0];
         if (specialised) {
-            // sorted row and channel tables for the specialised kernel's sample preparation
-            const size_t nfc = nspw * C;
-            HIP_TRY(g.rowrec.ensure(std::max<size_t>(1, rows) * sizeof(RowRec)));
-            HIP_TRY(g.chanrec.ensure(std::max<size_t>(1, nfc) * sizeof(ChanRec)));
-            a.rowrec = g.rowrec.as<RowRec>();
-            a.chanrec = g.chanrec.as<ChanRec>();
-            a.chan_lds = nfc <= CHAN_LDS_MAX ? 1u : 0u;
             const bool four = mode == MODE_4 || mode == MODE_4J;
             a.pol_contig = (four && !psf && p->number_of_polarization_terms == 4 &&
                             ((uintptr_t)a.vis % 16) == 0 && ((uintptr_t)a.weights % 16) == 0 &&
                             ((uintptr_t)a.flags % 4) == 0) ? 1u : 0u;
-            HIP_TRY(build_row_tables(a, (uint32_t)rows, mode == MODE_4J && rows > 0,
-                                     g.rowrec.as<RowRec>(), g.chanrec.as<ChanRec>(), g.stream));
         }
         HIP_TRY(hipEventRecord(tc.k0, g.stream));
         if (specialised)
