@@ -585,7 +585,7 @@ int build_plan(gridding_parameters *p, int mode, const bullseye_batch_facet *fac
     // for all (the shared arguments carry no scale of their own)
     a.u_scale = (double)((double)facets[0].nx * facets[0].cell_x * (double)ARCSEC_TO_RAD);
     a.v_scale = -(double)((double)facets[0].ny * facets[0].cell_y * (double)ARCSEC_TO_RAD);
-    HIP_TRY(pl.walk.ensure(sizeof(int)));
+    HIP_TRY(pl.walk.ensure(walk_order_bytes()));
     a.time_inner = pl.walk.as<int>();
     HIP_TRY(launch_walk_order(a, lv.walk_order, pl.walk.as<int>(), lv.stream));
     // sorted row table for the class kernels' sample preparation (rebuilt by every pass, see

@@ -23,6 +23,8 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
                         uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
                         hipStream_t st);
 int romein_max_width();
+// time_inner: the flag followed by the measurement's partial sums (walk_order_bytes() in all)
+size_t walk_order_bytes();
 hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
                              hipStream_t st);
 hipError_t launch_degrid(const GridArgs &a, const void *model, void *out, uint64_t rows,

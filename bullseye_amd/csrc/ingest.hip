@@ -85,22 +85,25 @@ hipError_t ingest_sort(const uint32_t *a1, const uint32_t *a2, uint32_t rows, ui
 // Walk order of the Romein streams: the mean footprint movement, |du| + |dv| in grid cells,
 // per time step (consecutive rows of a baseline at the middle channel) and per channel step
 // (the middle channel pair), over up to 4096 baselines x 32 rows.  Time inner when a time step
-// moves the footprint less: that order flushes fewer grid points.  One workgroup; the result
-// stays on the device for the gridding kernels.  override_order: 1 channels inner, 2 time
-// inner, anything else measured.
-constexpr int WALK_THREADS = 1024;
+// moves the footprint less: that order flushes fewer grid points.  WALK_BLOCKS workgroups each
+// sum a share of the baselines into their own partial (no atomics); one thread then adds the
+// partials in block order, so the result is the same on every run.  The flag stays on the
+// device for the gridding kernels.  override_order: 1 channels inner, 2 time inner, anything
+// else measured.
+constexpr int WALK_BLOCKS = 64;
+constexpr int WALK_THREADS = 256;
 
-__global__ __launch_bounds__(WALK_THREADS) void k_walk_order(GridArgs a, int override_order,
-                                                             int *time_inner) {
+__global__ __launch_bounds__(WALK_THREADS) void k_walk_partial(GridArgs a, double *part) {
     __shared__ double s_d[2][WALK_THREADS];
     __shared__ uint32_t s_n[2][WALK_THREADS];
     double dt = 0, dc = 0;
     uint32_t nt = 0, nc = 0;
-    if (override_order != 1 && override_order != 2 && a.chan > 1) {
+    if (a.chan > 1) {
         const uint32_t B = a.baselines;
         const uint32_t nbs = B < 4096u ? B : 4096u;
         const uint32_t cm = (a.chan - 1) / 2, cn = cm + 1;
-        for (uint32_t i = threadIdx.x; i < nbs; i += WALK_THREADS) {
+        for (uint32_t i = blockIdx.x * WALK_THREADS + threadIdx.x; i < nbs;
+             i += WALK_BLOCKS * WALK_THREADS) {
             const uint32_t b = (uint32_t)((uint64_t)i * B / nbs);
             const uint32_t r0 = a.bl_start[b];
             const uint32_t r1 = min(a.bl_start[b + 1], r0 + 33);
@@ -141,13 +144,30 @@ __global__ __launch_bounds__(WALK_THREADS) void k_walk_order(GridArgs a, int ove
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        int ti = 0;
-        if (override_order == 2)
-            ti = 1;
-        else if (override_order != 1 && s_n[0][0] && s_n[1][0])
-            ti = s_d[0][0] / s_n[0][0] < s_d[1][0] / s_n[1][0] ? 1 : 0;
-        *time_inner = ti;
+        double *o = part + 4 * blockIdx.x;
+        o[0] = s_d[0][0];
+        o[1] = s_d[1][0];
+        o[2] = (double)s_n[0][0];
+        o[3] = (double)s_n[1][0];
     }
+}
+
+__global__ void k_walk_decide(const double *part, int override_order, int *time_inner) {
+    if (threadIdx.x != 0) return;
+    int ti = 0;
+    if (override_order == 2) {
+        ti = 1;
+    } else if (override_order != 1) {
+        double dt = 0, dc = 0, nt = 0, nc = 0;
+        for (int b = 0; b < WALK_BLOCKS; ++b) {
+            dt += part[4 * b];
+            dc += part[4 * b + 1];
+            nt += part[4 * b + 2];
+            nc += part[4 * b + 3];
+        }
+        if (nt > 0 && nc > 0) ti = dt / nt < dc / nc ? 1 : 0;
+    }
+    *time_inner = ti;
 }
 
 // Sorted row table: row i of the pass is row perm[i] of the arrays.  Jones modes also range-check
@@ -210,9 +230,14 @@ hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
     return hipGetLastError();
 }
 
+size_t walk_order_bytes() { return 256 + 4 * WALK_BLOCKS * sizeof(double); }
+
 hipError_t launch_walk_order(const GridArgs &a, int override_order, int *time_inner,
                              hipStream_t st) {
-    k_walk_order<<<1, WALK_THREADS, 0, st>>>(a, override_order, time_inner);
+    double *part = reinterpret_cast<double *>(reinterpret_cast<char *>(time_inner) + 256);
+    if (override_order != 1 && override_order != 2)
+        k_walk_partial<<<WALK_BLOCKS, WALK_THREADS, 0, st>>>(a, part);
+    k_walk_decide<<<1, 64, 0, st>>>(part, override_order, time_inner);
     return hipGetLastError();
 }
 

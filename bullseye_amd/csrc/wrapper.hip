@@ -464,7 +464,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.n_chan_blocks = (uint32_t)((C + cps - 1) / cps);
     a.n_segments = nseg;
     a.n_streams = (uint64_t)B * a.n_chan_blocks * nseg * g.F;
-    HIP_TRY(g.walk.ensure(sizeof(int)));
+    HIP_TRY(g.walk.ensure(walk_order_bytes()));
     a.time_inner = g.walk.as<int>();
     HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, g.walk.as<int>(), g.stream));
 
