@@ -197,7 +197,9 @@ typedef struct bullseye_options {
     uint32_t compute_path;      /* enum bullseye_compute_path */
     uint32_t apply_taper_correction; /* 1: finalize divides each facet image by the grid
                                    taper (DFT of the sampled AA kernel) */
-    uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto) */
+    uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto: up to
+                                     32 when every enabled channel maps to one grid slice, else 1;
+                                     w-projection streams take up to 8) */
     uint32_t rows_per_stream;   /* Romein stream: time rows walked per work item (0 = auto) */
     uint32_t walk_order;        /* order of a stream's samples: 0 = whichever moves the kernel
                                    footprint less per step, measured from the data; 1 =
