@@ -421,6 +421,9 @@ template <int MODE, int PATH, int W> struct GridderW {
 
         for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
             const uint32_t s = step0 + gl;
+            // issue priority: low while preparing (mostly waiting on loads), high while walking
+            // (a dense VALU/LDS stream that the other waves' waits can then interleave with)
+            __builtin_amdgcn_s_setprio(0);
             {
                 R_t rec;
                 rec.cgi = -1;
@@ -455,6 +458,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                 }
                 grp[gl] = rec;
             }
+            __builtin_amdgcn_s_setprio(3);
             wave_sync();
             const int kmax = (int)min((uint32_t)G, nsamp - step0);
             int kk = 0;
