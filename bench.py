@@ -45,13 +45,17 @@ def parse():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before timing, copy this many MiB device to device once (a plain copy "
                          "of known size that scales HBM byte counters)")
-    ap.add_argument("--workload", choices=["faceting", "wproj", "adjoint", "batch"],
+    ap.add_argument("--workload", choices=["faceting", "plain", "wproj", "adjoint", "batch"],
                     default="faceting",
-                    help="faceting: the 25-facet Jones workload (default); wproj: the 4096^2 "
+                    help="faceting: the 25-facet Jones workload (default); plain: one 1024^2 "
+                         "grid of 64 ant x 240 t x 16 ch, 1 correlation (the shape of the "
+                         "reference's own benchmark.cpp); wproj: the 4096^2 "
                          "w-projection workload; adjoint: chi-squared value and gradient of a "
                          "2048^2 facet model through gridding and its degridding adjoint; "
                          "batch: the w-projection visibilities into 200 uneven facets in one "
                          "persistent graph launch")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="faceting: leave out the batched-facets figures of the default line")
     ap.add_argument("--batch-facets", type=int, default=200)
     ap.add_argument("--batch-timesteps", type=int, default=480,
                     help="timesteps of the w-projection visibilities the batch grids")
@@ -134,12 +138,64 @@ def rel_errors(got, want):
     return max(e[0] for e in out), max(e[1] for e in out)
 
 
+def facet_images(grids, taper=None):
+    """Float64 images of grids [F, P, ny, nx] by the finalize chain of
+    fft_and_repacking_routines.cpp:83-118 (ifftshift, unnormalised backward FFT, fftshift, real
+    part), optionally divided by the separable grid taper T(l) T(m)."""
+    import numpy as np
+    g = np.asarray(grids, dtype=np.complex128)
+    ny, nx = g.shape[-2:]
+    img = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(g, axes=(-2, -1)), axes=(-2, -1)),
+                          axes=(-2, -1)).real * (nx * ny)
+    if taper is not None:
+        img = img / np.outer(taper, taper)
+    return img
+
+
+def facet_error_table(got, want, norm_got, norm_want, taper):
+    """Per-facet error of `got` against `want` (grids [F, P, ny, nx], normalisation sums
+    [F * P]), relative to each facet's peak: (i) the normalised uv-grids, (ii) the images after
+    the same float64 shift / FFT / taper on both sides, over the central half of each axis (the
+    taper falls towards 1e-4 at the edges, where dividing by it only amplifies rounding)."""
+    import numpy as np
+    F, P, ny, nx = want.shape
+    ng = np.asarray(norm_got, np.float64).reshape(F, P, 1, 1)
+    nw = np.asarray(norm_want, np.float64).reshape(F, P, 1, 1)
+    gg = np.asarray(got, np.complex128) / ng
+    gw = np.asarray(want, np.complex128) / nw
+    cy, cx = slice(ny // 4, ny - ny // 4), slice(nx // 4, nx - nx // 4)
+    rows = {"grid_rms": [], "grid_peak": [], "image_rms": [], "image_peak": []}
+    for f in range(F):
+        d = np.abs(gg[f] - gw[f])
+        pk = np.abs(gw[f]).max()
+        rows["grid_rms"].append(float(np.sqrt(np.mean(d ** 2)) / pk))
+        rows["grid_peak"].append(float(d.max() / pk))
+        ig = facet_images(gg[f], taper)[:, cy, cx]
+        iw = facet_images(gw[f], taper)[:, cy, cx]
+        di = np.abs(ig - iw)
+        pi = np.abs(iw).max()
+        rows["image_rms"].append(float(np.sqrt(np.mean(di ** 2)) / pi))
+        rows["image_peak"].append(float(di.max() / pi))
+    out = {k: [float("%.3g" % x) for x in v] for k, v in rows.items()}
+    out["worst"] = {k: max(v) for k, v in out.items()}
+    return out
+
+
+def norm_terms(binding, n):
+    """The normalisation sums the library published into the caller's struct."""
+    import numpy as np
+    return np.ctypeslib.as_array(ctypes.cast(binding.params.normalization_terms,
+                                             ctypes.POINTER(ctypes.c_double)), (n,)).copy()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported; the facet-sharded multi-GPU layout is a "
               "design, not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "plain":
+        return main_plain(args)
     if args.workload == "wproj":
         return main_wproj(args)
     if args.workload == "adjoint":
@@ -205,15 +261,21 @@ def main():
         wall = (time.perf_counter() - t) / steps
         return wall, taps, float(np.mean(kms)), stats.samples_gridded
 
+    F, npx = im["facets"], im["npix"]
     wall32, taps, k32, nsamp = run(PATH_FP32_KAHAN, args.steps, args.warmup)
     grid32 = b.host("output_buffer").copy()
+    norm32 = norm_terms(b, F * 4)
     log("FP32-comp: %.2f ms/step, kernel %.2f ms, %.3e taps" % (wall32 * 1e3, k32, taps))
     wall64, taps64, k64, _ = run(PATH_FP64, max(1, args.steps // 2), 1)
     grid64 = b.host("output_buffer").copy()
+    norm64 = norm_terms(b, F * 4)
     log("FP64: %.2f ms/step, kernel %.2f ms" % (wall64 * 1e3, k64))
     lib.releaseLibrary()
-    rms_vs_gpu64, peak_vs_gpu64 = rel_errors(grid32.reshape(im["facets"], -1),
-                                             grid64.reshape(im["facets"], -1))
+    rms_vs_gpu64, peak_vs_gpu64 = rel_errors(grid32.reshape(F, -1), grid64.reshape(F, -1))
+    taper = kernels.grid_taper(conv, im["support"], im["oversample"], npx)
+    err_gpu = facet_error_table(grid32.reshape(F, 4, npx, npx), grid64.reshape(F, 4, npx, npx),
+                                norm32, norm64, taper)
+    del grid64
     # the reference's model: host arrays handed over every call (inputs copied to the device,
     # grids copied back on the final chunk); reported beside `value`, never as it
     bh = Binding("mixed", obs, spec)
@@ -253,6 +315,7 @@ def main():
         "host_buffers_ms_per_step": round(wall_host * 1e3, 3),
         "host_buffers_gtaps_per_s": round(taps / wall_host / 1e9, 3),
         "rms_err_vs_fp64_gpu": rms_vs_gpu64, "peak_err_vs_fp64_gpu": peak_vs_gpu64,
+        "error_per_facet_vs_fp64_gpu": err_gpu,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
@@ -271,7 +334,7 @@ def main():
             obs, im["facets"], np.complex128)})
         ref = Binding("double", obs, spec64, rows=sub)
         t = time.perf_counter()
-        oracle.run(ref, "facet_4_cor_corrections", normalize=False)
+        norm_or = oracle.run(ref, "facet_4_cor_corrections", normalize=False)
         t_or64 = time.perf_counter() - t
         # GPU FP32-comp on the same rows
         set_opts(PATH_FP32_KAHAN)
@@ -282,9 +345,14 @@ def main():
         lib.gridding_barrier()
         lib.bullseye_get_stats(ctypes.byref(stats))
         sample_taps = stats.taps
+        norm_s = norm_terms(bs, F * 4)
         lib.releaseLibrary()
-        rms_o, peak_o = rel_errors(bs.host("output_buffer").reshape(im["facets"], -1),
-                                   ref.output.reshape(im["facets"], -1))
+        gs = bs.host("output_buffer")
+        rms_o, peak_o = rel_errors(gs.reshape(F, -1), ref.output.reshape(F, -1))
+        result["error_per_facet_vs_fp64_oracle"] = facet_error_table(
+            gs.reshape(F, 4, npx, npx), ref.output.reshape(F, 4, npx, npx), norm_s, norm_or,
+            taper)
+        del gs
         # CPU baseline: the single-precision restatement on the same sample, all host cores
         ref32 = Binding("single", obs, spec, rows=sub)
         t = time.perf_counter()
@@ -300,6 +368,144 @@ def main():
                       "%.1f s; OpenMP over facets (reference scheme)" % (
                           n, obs.rows, n // 2016, sample_taps, t_or32)}
         result["fp64_oracle_cpu_gtaps_per_s"] = round(sample_taps / t_or64 / 1e9, 4)
+        del ref, ref32, bs
+    if not args.no_batch:
+        # the batched uneven-facet workload (pattern 3's target configuration), one measured
+        # replay of each schedule after one warm-up replay
+        del b, obs
+        torch.cuda.empty_cache()
+        r = batch_workload(args, 1, 1, True, False, False)
+        result["batched_facets"] = {
+            "workload": r["config"]["workload"], "facets": r["config"]["facets"],
+            "taps_per_step": r["config"]["taps_per_step"], "gtaps_per_s": r["value"],
+            "ms_per_step": r["ms_per_step"], "fp64_gtaps_per_s": r["fp64_gtaps_per_s"],
+            "fp32comp_over_fp64": r["fp32comp_over_fp64"],
+            "per_facet_launch_gtaps_per_s": r["per_facet_launch_gtaps_per_s"],
+            "batched_over_per_facet": r["batched_over_per_facet"],
+            "rms_err_vs_fp64_gpu_worst_facet": r["rms_err_vs_fp64_gpu_worst_facet"],
+            "peak_err_vs_fp64_gpu_worst_facet": r["peak_err_vs_fp64_gpu_worst_facet"],
+            "roofline_frac": r["roofline"]["frac"], "flush_atomics": r["flush_atomics"]}
+    print(json.dumps(result), flush=True)
+
+
+def main_plain(args):
+    """Plain single-facet gridding, the shape of the reference's benchmark
+    (bullseye/mo/benchmarks/benchmark.cpp:97-289, one grid_single_pol facet): 64 antennas
+    uniform in a 3 km disc (2016 baselines), 240 timesteps over 8 h at dec -30 deg, 16 channels
+    1.40-1.42 GHz, 1 correlation complex64 (20 point sources + noise, 5% flags, natural
+    weights); one 1024^2 grid at 2", 7x7 PSWF oversampled x63, no w-projection.  The whole
+    workload is also gridded by the FP64 CPU oracle (one thread: one grid) for the error, and by
+    the single-precision restatement for the CPU baseline."""
+    import numpy as np
+    import torch
+    from bullseye_amd import kernels, synth
+    from bullseye_amd.binding import Binding, ImagingSpec
+    from bullseye_amd.library_loader import (PATH_FP32_KAHAN, PATH_FP64, bullseye_options,
+                                             bullseye_stats, load_library)
+    cfg = synth.CONFIGS["c0"]
+    im = cfg["image"]
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    obs = synth.simulate(**cfg["sim"], seed=args.seed, device=dev)
+    conv = kernels.pswf_aa_kernel(im["support"], im["oversample"])
+    spec = ImagingSpec(npix=im["npix"], cell=im["cell"], conv=conv, support=im["support"],
+                       oversample=im["oversample"])
+    log("synthesised %d rows x %d ch in %.1fs" % (obs.rows, obs.n_chan, time.time() - t0))
+    lib = load_library("GPU", "mixed")
+    b = Binding("mixed", obs, spec, device=dev)
+    torch.cuda.synchronize()
+
+    def check(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (what, lib.bullseye_last_error().decode()))
+
+    def set_opts(path):
+        o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=1,
+                             compute_path=path, channels_per_stream=args.channels_per_stream,
+                             rows_per_stream=args.rows_per_stream)
+        check(lib.bullseye_set_options(ctypes.byref(o)), "options")
+
+    set_opts(PATH_FP32_KAHAN)
+    check(lib.initLibrary(b.byref()), "initLibrary")
+    stats = bullseye_stats()
+    calibration_copy(args.calibration_copy_mib)
+
+    def run(path, steps, warmup):
+        set_opts(path)
+        for _ in range(warmup):
+            check(lib.bullseye_reset_grids(), "reset")
+            check(lib.grid_single_pol(b.byref()), "grid")
+            lib.gridding_barrier()
+        torch.cuda.synchronize()
+        kms = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            check(lib.bullseye_reset_grids(), "reset")
+            check(lib.grid_single_pol(b.byref()), "grid")
+            lib.gridding_barrier()
+            lib.bullseye_get_stats(ctypes.byref(stats))
+            kms.append(stats.kernel_ms)
+        torch.cuda.synchronize()
+        return ((time.perf_counter() - t) / steps, stats.taps, float(np.mean(kms)),
+                stats.samples_gridded)
+
+    steps = max(args.steps, 20)
+    wall32, taps, k32, nsamp = run(PATH_FP32_KAHAN, steps, max(args.warmup, 3))
+    grid32 = b.host("output_buffer").copy()
+    norm32 = norm_terms(b, 1)
+    log("FP32-comp: %.3f ms/step, kernel %.3f ms, %.3e taps" % (wall32 * 1e3, k32, taps))
+    wall64, taps64, k64, _ = run(PATH_FP64, steps, 3)
+    grid64 = b.host("output_buffer").copy()
+    norm64 = norm_terms(b, 1)
+    log("FP64: %.3f ms/step, kernel %.3f ms" % (wall64 * 1e3, k64))
+    lib.releaseLibrary()
+    npx = im["npix"]
+    taper = kernels.grid_taper(conv, im["support"], im["oversample"], npx)
+    shape = (1, 1, npx, npx)
+    # algorithmic bytes of one pass: 13 B per sample (c64 visibility, f32 weight, bool flag),
+    # 33 B per row (f64 uvw, spw, field, row flag) and one complex64 grid write
+    alg_bytes = obs.rows * obs.n_chan * 13 + obs.rows * 33 + npx * npx * 8
+    result = {
+        "metric": METRIC, "value": round(taps / wall32 / 1e9, 3), "unit": "Gtaps/s",
+        "n_gpus": 1, "steps": steps, "warmup": max(args.warmup, 3),
+        "ms_per_step": round(wall32 * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded; inputs resident in HBM)",
+        "config": {"workload": "plain single-facet gridding: 64 ant (2016 baselines) x 240 t x "
+                               "16 ch (1.40-1.42 GHz), 1 corr, one 1024^2 grid at 2\", 7x7 PSWF "
+                               "x63, no w-projection (grid_single_pol)",
+                   "rows": obs.rows, "channels": obs.n_chan, "taps_per_step": taps,
+                   "samples_per_step": nsamp},
+        "fp64_gtaps_per_s": round(taps64 / wall64 / 1e9, 3),
+        "fp32comp_over_fp64": round(wall64 / wall32, 3),
+        "error_vs_fp64_gpu": facet_error_table(grid32.reshape(shape), grid64.reshape(shape),
+                                               norm32, norm64, taper)["worst"],
+        "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (k32 * 1e-3) / 1e9, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg_bytes / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                     "traffic": profiled_traffic("romein_w_grid_single_pol_plain_pmc.md"),
+                     "kernel_ms": round(k32, 4), "bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        ref = Binding("double", obs, spec)
+        t = time.perf_counter()
+        norm_or = oracle.run(ref, "grid_single_pol", normalize=False)
+        t_or64 = time.perf_counter() - t
+        result["error_vs_fp64_oracle"] = facet_error_table(
+            grid32.reshape(shape), ref.output.reshape(shape), norm32, norm_or, taper)["worst"]
+        result["fp64_path_error_vs_fp64_oracle"] = facet_error_table(
+            grid64.reshape(shape), ref.output.reshape(shape), norm64, norm_or, taper)["worst"]
+        del ref
+        ref32 = Binding("single", obs, spec)
+        t = time.perf_counter()
+        oracle.run(ref32, "grid_single_pol", precision="single", normalize=False)
+        t_or32 = time.perf_counter() - t
+        result["cpu_baseline"] = {
+            "value": round(taps / t_or32 / 1e9, 4), "unit": "Gtaps/s", "cores": 1,
+            "kind": "port",
+            "sample": "the whole workload, %.2e taps, %.1f s; one thread (the reference "
+                      "parallelises over facets and this workload has one)" % (taps, t_or32)}
+        result["fp64_oracle_cpu_gtaps_per_s"] = round(taps / t_or64 / 1e9, 4)
     print(json.dumps(result), flush=True)
 
 
@@ -539,6 +745,11 @@ def main_adjoint(args):
     print(json.dumps(result), flush=True)
 
 def main_batch(args):
+    print(json.dumps(batch_workload(args, args.steps, args.warmup, not args.no_compare, True,
+                                    not args.no_cpu)), flush=True)
+
+
+def batch_workload(args, steps, warmup, compare_fp64, compare_tiles, cpu):
     """Batched uneven facets: the w-projection workload's visibilities (128 antennas, 10 km
     log-radial, 32 channels 1.0-1.5 GHz, 1 correlation; --batch-timesteps of its 480 timesteps)
     gridded into --batch-facets facets with sizes log-uniform over 128^2-2048^2 at 1.5", each
@@ -653,11 +864,11 @@ def main_batch(args):
             what, wall * 1e3, tplan, taps, taps / wall / 1e9))
         return wall, taps, float(np.mean(kms)), stats.samples_gridded, tplan
 
-    wall32, taps, k32, nsamp, tplan = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, args.steps,
-                                          args.warmup, "persistent graph, tuned")
+    wall32, taps, k32, nsamp, tplan = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, steps,
+                                          warmup, "persistent graph, tuned")
     tuned = batch.tuning()
     compare = {}
-    if not args.no_compare:
+    if compare_fp64:
         out32 = [o.clone() for o in batch.outputs]
         wall64, taps64, _, _, _ = run(PATH_FP64, BATCH_PERSISTENT_GRAPH, 1, 0, "FP64 persistent")
         errs = []
@@ -668,27 +879,30 @@ def main_batch(args):
             if peak > 0:
                 errs.append((torch.sqrt((d ** 2).mean()).item() / peak, d.max().item() / peak))
         del out32
-        batch.set_tuning()
-        wall_d, taps_d, _, _, _ = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, 1, 0,
-                                      "persistent graph, default tiles")
+        # one launch per (facet, class) of the single-facet kernels, with the tuned tiles
         wall_pf, taps_pf, _, _, _ = run(PATH_FP32_KAHAN, BATCH_PER_FACET, 1, 0,
                                         "one launch per facet and class")
-        assert taps_d == taps and taps_pf == taps and taps64 == taps
+        assert taps_pf == taps and taps64 == taps
         compare = {
             "fp64_gtaps_per_s": round(taps64 / wall64 / 1e9, 3),
             "fp32comp_over_fp64": round(wall64 / wall32, 3),
-            "default_tiles_gtaps_per_s": round(taps_d / wall_d / 1e9, 3),
             "per_facet_launch_gtaps_per_s": round(taps_pf / wall_pf / 1e9, 3),
             "batched_over_per_facet": round(wall_pf / wall32, 3),
             "rms_err_vs_fp64_gpu_worst_facet": max(e[0] for e in errs),
             "peak_err_vs_fp64_gpu_worst_facet": max(e[1] for e in errs)}
+        if compare_tiles:
+            batch.set_tuning()
+            wall_d, taps_d, _, _, _ = run(PATH_FP32_KAHAN, BATCH_PERSISTENT_GRAPH, 1, 0,
+                                          "persistent graph, default tiles")
+            assert taps_d == taps
+            compare["default_tiles_gtaps_per_s"] = round(taps_d / wall_d / 1e9, 3)
     batch.release()
     lib.releaseLibrary()
     flops = taps * 8
     n_psf = int(psf.sum())
     result = {
         "metric": METRIC, "value": round(taps / wall32 / 1e9, 3), "unit": "Gtaps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(wall32 * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded; inputs resident in HBM)",
         "config": {"workload": "batched uneven facets: w-projection visibilities (128 ant, 8128 "
@@ -718,7 +932,7 @@ def main_batch(args):
         result["autotune"] = {"timesteps": args.tune_timesteps,
                               "tuned_ms": round(tune["tuned_ms"], 3),
                               "default_ms": round(tune["default_ms"], 3)}
-    if not args.no_cpu:
+    if cpu:
         # one facet of the batch on the first two timesteps: its GPU tap count through the
         # single-facet entry point, then the oracle's single-precision build on the same rows
         from oracle import oracle
@@ -751,7 +965,7 @@ def main_batch(args):
                       "%.2e in-support taps, %.1f s; one thread (the reference parallelises over "
                       "the facets of one geometry); the oracle visits the full 63x63 support"
                       % (cpu_f.npix, P, args.batch_timesteps, sample_taps, t_or32)}
-    print(json.dumps(result), flush=True)
+    return result
 
 
 if __name__ == "__main__":

@@ -20,8 +20,21 @@ namespace bam {
 
 
 
+// Two timing events released on every return path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create() {
+        hipError_t e = hipEventCreate(&e0);
+        return e != hipSuccess ? e : hipEventCreate(&e1);
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 struct TimedCall {
-    hipEvent_t start, k0, k1, stop;
+    hipEvent_t start, copied, k0, k1, stop;
     bool has_kernel;
     bool recorded;   // every event recorded (a call that failed midway is only released)
 };
@@ -144,13 +157,15 @@ static int barrier_impl() {
     for (auto &c : g.pending) {
         float ms = 0, kms = 0;
         if (!c.recorded) {
-            for (hipEvent_t ev : {c.start, c.k0, c.k1, c.stop})
+            for (hipEvent_t ev : {c.start, c.copied, c.k0, c.k1, c.stop})
                 if (ev) (void)hipEventDestroy(ev);
             continue;
         }
         (void)hipEventElapsedTime(&ms, c.start, c.stop);
         g.grid_seconds += ms * 1e-3;
-        g.stats.copy_ms = 0;
+        float cms = 0;
+        (void)hipEventElapsedTime(&cms, c.start, c.copied);
+        g.stats.copy_ms = dev_mode() ? 0.0 : cms;
         if (c.has_kernel) {
             (void)hipEventElapsedTime(&kms, c.k0, c.k1);
             g.stats.kernel_ms = kms;
@@ -159,6 +174,7 @@ static int barrier_impl() {
             g.stats.prep_ms = pms;
         }
         (void)hipEventDestroy(c.start);
+        (void)hipEventDestroy(c.copied);
         (void)hipEventDestroy(c.k0);
         (void)hipEventDestroy(c.k1);
         (void)hipEventDestroy(c.stop);
@@ -251,8 +267,9 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
 // Inputs of a gridding or degridding call on the device (staged copies, or the caller's own
 // device pointers), the inverted Jones terms, the per-facet geometry and the scalar fields of
 // GridArgs that do not depend on the stream decomposition.
+// `copied` (optional) is recorded on the stream once the staging copies are queued.
 static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase, bool psf,
-                        bool need_vis, GridArgs &a) {
+                        bool need_vis, GridArgs &a, hipEvent_t copied = nullptr) {
     const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
     const size_t nspw = std::max<size_t>(1, p->spw_count);
     const void *d;
@@ -287,16 +304,19 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
     a.cgi = (const uint64_t *)d;
     // per-row arrays may be NULL in an empty chunk (the reference's loops then never run)
     const bool any_rows = rows > 0;
+    const size_t nj = p->no_timestamps_read * p->antenna_count * g.F * nspw * C;
+    if (mode == MODE_4J && any_rows) {
+        HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
+        a.time_ids = (const uint64_t *)d;
+        HIP_TRY(stage(g.in_jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d));
+        a.jones = (const visibility_base_type *)d;
+    }
+    if (copied) HIP_TRY(hipEventRecord(copied, g.stream));
     if (!a.wavelengths || !a.enabled || !a.cgi ||
         (any_rows && (!a.uvw || !a.row_flags || !a.field || !a.spw ||
                       (!psf && ((need_vis && !a.vis) || !a.weights || !a.flags)))))
         return fail("a required array pointer is NULL");
     if (mode == MODE_4J && any_rows) {
-        HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
-        a.time_ids = (const uint64_t *)d;
-        const size_t nj = p->no_timestamps_read * p->antenna_count * g.F * nspw * C;
-        HIP_TRY(stage(g.in_jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d));
-        a.jones = (const visibility_base_type *)d;
         if (!a.time_ids) return fail("timestamp_ids is NULL");
         // jones_2x2.h:113-126 invert_all, into a library buffer in the compute precision
         const bool fp64 = g.path == PATH_FP64;
@@ -413,13 +433,14 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     g.pending.push_back(TimedCall{});
     TimedCall &tc = g.pending.back();
     HIP_TRY(hipEventCreate(&tc.start));
+    HIP_TRY(hipEventCreate(&tc.copied));
     HIP_TRY(hipEventCreate(&tc.k0));
     HIP_TRY(hipEventCreate(&tc.k1));
     HIP_TRY(hipEventCreate(&tc.stop));
     HIP_TRY(hipEventRecord(tc.start, g.stream));
 
     GridArgs a{};
-    if (setup_inputs(p, mode, facet, phase, psf, true, a)) return 1;
+    if (setup_inputs(p, mode, facet, phase, psf, true, a, tc.copied)) return 1;
 
     // ingest: stable sort of rows by baseline
     HIP_TRY(g.perm.ensure(rows * 4));
@@ -450,7 +471,10 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     }
     uint32_t cps, nseg;
     choose_streams(p, g.F, B, rows, W, mode, same_slice, cps, nseg);
-    if (wproj) cps = same_slice ? (uint32_t)std::min<size_t>(C, 8) : 1;
+    if (wproj)
+        cps = g.opts.channels_per_stream
+                  ? (uint32_t)std::max<size_t>(1, std::min<size_t>(g.opts.channels_per_stream, C))
+                  : same_slice ? (uint32_t)std::min<size_t>(C, 8) : 1;
 
     a.perm = g.perm.as<uint32_t>();
     a.bl_start = g.blstart.as<uint32_t>();
@@ -652,6 +676,14 @@ int bullseye_set_options(const bullseye_options *o) {
         return fail("bullseye_options.struct_size %u != %zu", o->struct_size, sizeof(bullseye_options));
     if (o->compute_path > BULLSEYE_PATH_FP32) return fail("bad compute_path %u", o->compute_path);
     if (o->walk_order > 2) return fail("bad walk_order %u", o->walk_order);
+    if (g.initialized) {
+        // initLibrary chose where the grids are seeded from and published to: the buffer
+        // model cannot change under queued or future calls of the same context
+        if ((o->device_resident != 0) != (g.opts.device_resident != 0))
+            return fail("device_resident cannot change between initLibrary and releaseLibrary");
+        // queued calls finish on the path they were launched with
+        if (barrier_impl()) return 1;
+    }
     g.opts = *o;
     if (g.initialized)
         g.path = o->compute_path ? (int)o->compute_path
@@ -702,12 +734,34 @@ double get_inversion_walltime(void) { return g.inv_seconds; }
 
 void gridding_barrier(void) { (void)barrier_impl(); }
 
-int initLibrary(gridding_parameters *p) {
-    if (g.initialized) return 0;
-    if (!p) return fail("NULL params");
-    g.err.clear();
+// Device buffers and the stream of the context (releaseLibrary, or a failed initLibrary).
+static void release_resources() {
+    DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv_f,
+                      &g.conv_d, &g.tapsum,
+                      &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
+                      &g.rowrec, &g.chanrec,
+                      &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
+                      &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
+                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
+                      &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
+                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
+                      &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
+    for (DevBuf *b : bufs) b->release();
+    batch_release_all();
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    g.stream = nullptr;
+}
+
+static int init_impl(gridding_parameters *p) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device");
+    if (p->nx == 0 || p->ny == 0 || p->conv_oversample == 0)
+        return fail("nx, ny and conv_oversample must be > 0");
+    // kernels form grid row strides (nx * 4 correlations * 2) in 32 bits
+    if (p->nx > (1u << 28) || p->ny > (1u << 28)) return fail("nx and ny must be <= 2^28");
+    if (2 * p->conv_support + 3 >= p->nx || 2 * p->conv_support + 3 >= p->ny)
+        return fail("conv_support too large for the grid");
+    if (!p->conv) return fail("conv is NULL");
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     g.F = std::max<size_t>(1, p->num_facet_centres);
     g.cube = std::max<size_t>(1, p->cube_channel_dim_size);
@@ -727,11 +781,6 @@ int initLibrary(gridding_parameters *p) {
 #else
                                  : PATH_F32K;
 #endif
-    if (g.nx == 0 || g.ny == 0 || g.OS == 0) return fail("nx, ny and conv_oversample must be > 0");
-    // kernels form grid row strides (nx * 4 correlations * 2) in 32 bits
-    if (g.nx > (1u << 28) || g.ny > (1u << 28)) return fail("nx and ny must be <= 2^28");
-    if (2 * g.S + 3 >= g.nx || 2 * g.S + 3 >= g.ny) return fail("conv_support too large for the grid");
-    if (!p->conv) return fail("conv is NULL");
     HIP_TRY(g.grid.ensure(grid_elems() * sizeof(double)));
     g.norm_stride = g.F * g.cube * g.P;
     g.nslots = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, ((size_t)1 << 22) / g.norm_stride));
@@ -827,6 +876,19 @@ int initLibrary(gridding_parameters *p) {
     HIP_TRY(hipStreamSynchronize(g.stream));
     g.grid_seconds = g.inv_seconds = 0;
     g.stats = bullseye_stats{};
+    return 0;
+}
+
+int initLibrary(gridding_parameters *p) {
+    if (g.initialized) return 0;
+    if (!p) return fail("NULL params");
+    g.err.clear();
+    if (init_impl(p)) {
+        // a failed set-up keeps nothing: the next initLibrary starts from scratch
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+        release_resources();
+        return 1;
+    }
     g.initialized = true;
     return 0;
 }
@@ -834,20 +896,7 @@ int initLibrary(gridding_parameters *p) {
 int releaseLibrary(void) {
     if (!g.initialized) return 0;
     (void)barrier_impl();
-    DevBuf *bufs[] = {&g.grid, &g.psfgrid, &g.norm, &g.counters, &g.errflag, &g.conv_f,
-                      &g.conv_d, &g.tapsum,
-                      &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
-                      &g.rowrec, &g.chanrec,
-                      &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
-                      &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
-                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
-                      &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
-                      &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
-                      &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
-    for (DevBuf *b : bufs) b->release();
-    batch_release_all();
-    (void)hipStreamDestroy(g.stream);
-    g.stream = nullptr;
+    release_resources();
     g.initialized = false;
     return 0;
 }
@@ -919,10 +968,9 @@ static int finalize_impl(gridding_parameters *p, bool psf) {
                                             : p->cube_channel_dim_size * g.P);
     const size_t npix = p->nx * p->ny;
     const size_t total_bytes = (size_t)F * fstride * npix * 2 * sizeof(grid_base_type);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, g.stream));
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e0, g.stream));
     void *dbuf = buf;
     if (!dev_mode()) {
         HIP_TRY(g.out_tmp.ensure(std::max(total_bytes, g.out_tmp.bytes)));
@@ -936,13 +984,11 @@ static int finalize_impl(gridding_parameters *p, bool psf) {
     const double *ty = taper ? g.taper.as<double>() + g.nx : nullptr;
     HIP_TRY(finalize_device(dbuf, F, n, fstride, (uint32_t)p->nx, (uint32_t)p->ny, tx, ty,
                             g.fft_scratch.p, g.fft_real.as<float>(), dev_mode(), g.stream));
-    HIP_TRY(hipEventRecord(e1, g.stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e1, g.stream));
+    HIP_TRY(hipEventSynchronize(ev.e1));
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, ev.e0, ev.e1);
     g.inv_seconds += ms * 1e-3;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (!dev_mode()) {
         HIP_TRY(hipMemcpy(buf, g.out_tmp.p, total_bytes, hipMemcpyDeviceToHost));
         // fft_and_repacking_routines.cpp:106-116: real parts as float32, in place
@@ -1116,9 +1162,8 @@ int bullseye_degrid(gridding_parameters *p, int mode, const void *model, void *v
     const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
     const size_t model_bytes = g.F * g.cube * g.P * g.ny * g.nx * 2 * sizeof(grid_base_type);
     const size_t out_bytes = rows * C * npol * 2 * sizeof(visibility_base_type);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_TRY(ev.create());
     GridArgs a{};
     if (setup_inputs(p, km, modes[mode].facet, modes[mode].phase, false, false, a)) return 1;
     const void *dm = model;
@@ -1132,19 +1177,17 @@ int bullseye_degrid(gridding_parameters *p, int mode, const void *model, void *v
     }
     const bool wproj = g.wplanes > 1;
     const void *taps = wproj ? (g.path == PATH_FP64 ? g.wtaps_d.p : g.wtaps_f.p) : nullptr;
-    HIP_TRY(hipEventRecord(e0, g.stream));
+    HIP_TRY(hipEventRecord(ev.e0, g.stream));
     HIP_TRY(launch_degrid(a, dm, dout, rows, (uint32_t)g.pstride, taps,
                           g.wplane_off.as<uint64_t>(), g.wplane_half.as<uint32_t>(),
                           (uint32_t)g.wplanes, g.w_nf, g.w_fbase, (float)p->wmax_est, km, g.path,
                           g.stream));
-    HIP_TRY(hipEventRecord(e1, g.stream));
+    HIP_TRY(hipEventRecord(ev.e1, g.stream));
     if (!dev_mode())
         HIP_TRY(hipMemcpyAsync(vis_out, dout, out_bytes, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, ev.e0, ev.e1);
     g.stats.kernel_ms = ms;
     return check_device_errors();
 }

@@ -198,8 +198,9 @@ typedef struct bullseye_options {
     uint32_t apply_taper_correction; /* 1: finalize divides each facet image by the grid
                                    taper (DFT of the sampled AA kernel) */
     uint32_t channels_per_stream; /* Romein stream: channels walked together (0 = auto: up to
-                                     32 when every enabled channel maps to one grid slice, else 1;
-                                     w-projection streams take up to 8) */
+                                     32 AA / 8 w-projection channels when every enabled channel
+                                     maps to one grid slice, else 1; a non-zero value is used
+                                     as given, capped at channel_count, by both gridders) */
     uint32_t rows_per_stream;   /* Romein stream: time rows walked per work item (0 = auto) */
     uint32_t walk_order;        /* order of a stream's samples: 0 = whichever moves the kernel
                                    footprint less per step, measured from the data; 1 =
@@ -225,8 +226,11 @@ typedef struct bullseye_stats {
     uint64_t samples_gridded;   /* (facet, row, channel) samples that were in-grid and unflagged */
     uint64_t taps;              /* sum over those samples of the tap count (2S+1)^2 */
     double kernel_ms;           /* device time of the gridding kernel(s) of the call */
-    double prep_ms;             /* device time of ingest (baseline sort) of the call */
-    double copy_ms;             /* device time of host->device copies of the call */
+    double prep_ms;             /* device time from the call's start to its gridding kernel:
+                                   staging copies, Jones inversion, ingest (baseline sort),
+                                   row tables and stream classification */
+    double copy_ms;             /* device time of the call's host->device staging copies
+                                   (0 with device_resident: nothing is copied) */
     uint32_t time_inner;        /* walk order of the call's streams: 1 time inner, 0 channels */
     uint32_t reserved;
 } bullseye_stats;

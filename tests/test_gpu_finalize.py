@@ -179,3 +179,80 @@ def test_repack_input_data_matches_reference_order():
                       ("visibility_weights", orig.weights), ("flags", orig.flags),
                       ("antenna_1_ids", orig.ant1), ("antenna_2_ids", orig.ant2)]:
         np.testing.assert_array_equal(b.keep[name], arr[p])
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("device", [False, True])
+def test_weight_uniformly_multi_slice(precision, device):
+    """Sampling function in four slices (one per channel), visibilities in two grid slices with
+    a non-trivial channel map, two facets: weight_uniformly (device kernel with device-resident
+    buffers, host loop otherwise) then normalize, against the oracle's chain
+    (cpu_algorithm/wrapper.cpp:116-163)."""
+    import torch
+    print("case uniform multi-slice", precision, device, flush=True)
+    mode = "facet_single_pol"
+    obs = round_to(small_obs(mode, seed=31, n_chan=4), precision)
+    spec = make_spec(obs, mode, precision=precision, npix=128, n_facets=2)
+    spec.cube = 2
+    spec.channel_grid_index = np.array([1, 0, 1, 1], np.uint64)
+    spec.sf_count = 4
+    spec.sf_channel_grid_index = np.arange(4, dtype=np.uint64)
+    ref = Binding("double", obs, spec, with_psf=True)
+    oracle.run(ref, "facet_sampling_function", normalize=False)
+    oracle.run(ref, mode, normalize=False, weight_uniformly=True)
+    oracle.load("double").oracle_normalize(ref.byref())
+    lib = load_library("GPU", precision)
+    set_options(lib, device_resident=int(device), compute_path=PATH_FP64)
+    dev = torch.device("cuda:0") if device else None
+    b = Binding(precision, obs, spec, device=dev, with_psf=True)
+    if device:
+        torch.cuda.synchronize()
+    try:
+        assert lib.initLibrary(b.byref()) == 0, lib.bullseye_last_error()
+        assert lib.facet_sampling_function(b.byref()) == 0, lib.bullseye_last_error()
+        assert getattr(lib, mode)(b.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        assert lib.weight_uniformly(b.byref()) == 0, lib.bullseye_last_error()
+        assert lib.normalize(b.byref()) == 0, lib.bullseye_last_error()
+    finally:
+        lib.releaseLibrary()
+        set_options(lib)
+    got = b.host("output_buffer")
+    rms, peak = rel_err(got, ref.output)
+    bar = (1e-11, 1e-9) if precision == "double" else (2e-7, 2e-6)
+    assert rms < bar[0] and peak < bar[1], (rms, peak)
+    # slice 0 is weighted by sampling-function channel 1 only, slice 1 by channels 0, 2 and 3
+    assert np.abs(ref.output[:, 0]).max() > 0 and np.abs(ref.output[:, 1]).max() > 0
+
+
+def test_device_resident_cannot_change_while_initialised():
+    """bullseye_set_options refuses a device_resident change between initLibrary and
+    releaseLibrary (the grids were seeded from, and publish to, one buffer model) and leaves
+    the options as they were; the context keeps gridding correctly."""
+    print("case options refusal", flush=True)
+    mode = "grid_single_pol"
+    obs = round_to(small_obs(mode, seed=32), "double")
+    spec = make_spec(obs, mode, precision="double", npix=128)
+    ref = Binding("double", obs, spec)
+    oracle.run(ref, mode)
+    lib = load_library("GPU", "double")
+    set_options(lib, device_resident=0)
+    b = Binding("double", obs, spec)
+    from bullseye_amd.library_loader import bullseye_options
+    try:
+        assert lib.initLibrary(b.byref()) == 0
+        o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=1)
+        assert lib.bullseye_set_options(ctypes.byref(o)) == 1
+        assert b"device_resident" in lib.bullseye_last_error()
+        cur = bullseye_options(struct_size=ctypes.sizeof(bullseye_options))
+        assert lib.bullseye_get_options(ctypes.byref(cur)) == 0
+        assert cur.device_resident == 0
+        # a compute-path change is accepted (queued work finishes on its own path first)
+        set_options(lib, device_resident=0, compute_path=PATH_FP64)
+        assert lib.grid_single_pol(b.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        assert lib.normalize(b.byref()) == 0
+    finally:
+        lib.releaseLibrary()
+        set_options(lib)
+    assert rel_err(b.output, ref.output)[0] < 1e-11

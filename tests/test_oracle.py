@@ -173,6 +173,66 @@ def test_wprojection_plane_choice():
     assert np.abs(b.output).max() == 0 and norm[0] == 0   # plane >= wplanes: dropped
 
 
+def wproj_closed_form(uc, vc, w, vis, conv, S2, OS2, wpl, wmax, n):
+    """The grid and normalisation one w-projection sample deposits, from the reference's own
+    arithmetic (cpu_algorithm/convolution_policies.h:247-287) written out independently: for
+    w < 0 the visibility is conjugated and (u, v, w) negated; origin lrint(u + n/2 - S) (nearest,
+    FE_TONEAREST), fraction size_t((1 - u + lrint(u)) * OS), plane lrint(|w|/wmax (W-1)); tap
+    (j, k) of the footprint is conv[plane][frac_v + j OS][frac_u + k OS] (row = v, column = u),
+    multiplied into the visibility unconjugated; the normalisation adds Re(tap)."""
+    if w < 0:
+        vis, uc, vc, w = np.conj(vis), -uc, -vc, -w
+    x0 = int(np.rint(uc + (n // 2 - S2)))
+    y0 = int(np.rint(vc + (n // 2 - S2)))
+    fu = int((1 - uc + np.rint(uc)) * OS2)
+    fv = int((1 - vc + np.rint(vc)) * OS2)
+    plane = int(np.rint(abs(w) / float(np.float32(wmax)) * (wpl - 1)))
+    k = np.arange(2 * S2 + 1)
+    taps = conv[plane][np.ix_(fv + OS2 * k, fu + OS2 * k)]
+    g = np.zeros((n, n), dtype=np.complex128)
+    g[y0:y0 + 2 * S2 + 1, x0:x0 + 2 * S2 + 1] = vis * taps
+    return g, taps.real.sum(), plane
+
+
+@pytest.mark.parametrize("uvw,vis", [([210.3, -75.6, 31.0], 0.3 + 0.8j),
+                                     ([-143.2, 96.45, 77.0], -1.1 + 0.25j),
+                                     ([118.9, 40.7, -58.0], 0.6 - 0.4j),
+                                     ([-96.1, -151.3, -12.0], 0.2 + 1.3j)])
+def test_wprojection_single_sample_closed_form(uvw, vis):
+    """One w-projection sample on a random asymmetric complex kernel stack lands exactly on the
+    closed form: a transposed tap index, a conjugated kernel, an |conv| normalisation or a missing
+    w < 0 conjugation in the oracle fails this."""
+    wpl, S2, OS2, n, wmax = 6, 2, 8, 64, 100.0
+    csz = kernels.aa_kernel_size(S2, OS2)
+    rng = np.random.default_rng(17)
+    conv = rng.normal(size=(wpl, csz, csz)) + 1j * rng.normal(size=(wpl, csz, csz))
+    weight = 0.75
+    obs = tiny_obs([uvw], [[vis]])
+    obs.weights[:] = weight
+    b, norm = grid_one(obs, conv=conv, support=S2, oversample=OS2, wplanes=wpl, wmax=wmax,
+                       npix=n)
+    sc = u_scale(n, 20.0)
+    uc, vc, v0 = uvw[0] * sc, -(uvw[1] * sc), np.complex128(np.complex64(vis)) * weight
+    want, tapsum, plane = wproj_closed_form(uc, vc, uvw[2], v0, conv, S2, OS2, wpl, wmax, n)
+    assert 0 < plane < wpl - 1
+    got = b.output[0, 0, 0]
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14)
+    assert norm[0] == pytest.approx(weight * tapsum, rel=1e-13)
+    # each plausible mistake gives a visibly different grid or normalisation on this stack
+    wrong = [wproj_closed_form(uc, vc, uvw[2], v0, conv.transpose(0, 2, 1), S2, OS2, wpl, wmax,
+                               n)[0],
+             wproj_closed_form(uc, vc, uvw[2], v0, np.conj(conv), S2, OS2, wpl, wmax, n)[0]]
+    if uvw[2] < 0:   # the w < 0 conjugation left out
+        wrong.append(wproj_closed_form(uc, vc, uvw[2], np.conj(v0), conv, S2, OS2, wpl, wmax,
+                                       n)[0])
+    for g2 in wrong:
+        assert np.abs(g2 - got).max() > 1e-3 * np.abs(got).max()
+    k = np.arange(2 * S2 + 1)
+    fu, fv = [int((1 - x + np.rint(x)) * OS2) for x in ((uc, vc) if uvw[2] > 0 else (-uc, -vc))]
+    abs_sum = np.abs(conv[plane][np.ix_(fv + OS2 * k, fu + OS2 * k)]).sum()
+    assert abs(weight * abs_sum - norm[0]) > 1e-3 * abs(norm[0])
+
+
 # ---- facet phase rotation and uvw transform ---------------------------------------------
 
 def facet_direction(ra0, dec0, ra, dec):
@@ -267,6 +327,50 @@ def test_normalize_and_weight_uniformly_closed_forms():
     weighted = b.output.copy()
     oracle.load("double").oracle_normalize(b.byref())
     np.testing.assert_allclose(b.output, weighted / norm[0], rtol=1e-14)
+
+
+def weight_uniformly_closed_form(out, psf, cgi):
+    """cpu_algorithm/wrapper.cpp:116-135 written out: for facet f, grid slice g and pixel,
+    count = 1e-7f + sum over sampling-function channels c of [cgi[c] == g] Re(psf[f][c]) -- the
+    visibility channel->grid map indexed by the sampling-function channel -- and every
+    correlation of slice g is multiplied by 1/count."""
+    F, G, P = out.shape[:3]
+    sfc = psf.shape[1]
+    res = out.copy()
+    for f in range(F):
+        for g in range(G):
+            count = np.full(out.shape[3:], np.float32(1e-7), dtype=np.float64)
+            for c in range(sfc):
+                if cgi[c] == g:
+                    count = count + psf[f, c, 0].real
+            res[f, g] *= 1.0 / count
+    return res
+
+
+def test_weight_uniformly_multi_slice_closed_form():
+    """Two facets, two grid slices, two correlations, four sampling-function channels whose
+    channel_grid_indicies entries differ from their sampling_function_channel_grid_indicies and
+    from the identity: the oracle must use channel_grid_indicies[c] for sampling-function
+    channel c, as the reference does."""
+    rng = np.random.default_rng(23)
+    obs = tiny_obs([[10.0, 5.0, 0.0]], [[1.0, 1.0, 1.0, 1.0]])
+    n = 16
+    spec = ImagingSpec(npix=n, cell=20.0, conv=CONV, support=S, oversample=OS,
+                       facet_centres=np.array([[162000.0, -108000.0], [162100.0, -108050.0]]),
+                       n_pol_gridded=2, cube=2, sf_count=4,
+                       channel_grid_index=np.array([1, 0, 1, 1], np.uint64),
+                       sf_channel_grid_index=np.array([0, 0, 1, 1], np.uint64))
+    b = Binding("double", obs, spec, with_psf=True)
+    b.output[...] = rng.normal(size=b.output.shape) + 1j * rng.normal(size=b.output.shape)
+    b.psf[...] = rng.uniform(0.5, 2.0, size=b.psf.shape) + 1j * rng.normal(size=b.psf.shape)
+    raw = b.output.copy()
+    oracle.load("double").oracle_weight_uniformly(b.byref())
+    cgi = np.array([1, 0, 1, 1])
+    np.testing.assert_allclose(b.output, weight_uniformly_closed_form(raw, b.psf, cgi),
+                               rtol=1e-13)
+    # the sampling-function map or the identity map in its place gives a different answer
+    for wrong in (np.array([0, 0, 1, 1]), np.arange(4)):
+        assert np.abs(weight_uniformly_closed_form(raw, b.psf, wrong) - b.output).max() > 1e-3
 
 
 def test_baseline_index_and_stable_order():
