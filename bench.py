@@ -529,7 +529,7 @@ def main_wproj(args):
     halves = kernels.linear_plane_supports(im["wplanes"], 3, im["support"])
     conv = kernels.wproj_kernels(im["support"], im["oversample"], im["wplanes"], wmax,
                                  im["npix"], im["cell"], half_supports=halves,
-                                 dtype=np.complex64)
+                                 dtype=np.complex64, device=dev)
     spec = ImagingSpec(npix=im["npix"], cell=im["cell"], conv=conv, support=im["support"],
                        oversample=im["oversample"], wplanes=im["wplanes"], wmax=wmax)
     log("synthesised %d rows x %d ch and %d w-kernels in %.1fs" % (obs.rows, obs.n_chan,
@@ -655,32 +655,14 @@ def main_adjoint(args):
     device."""
     import numpy as np
     import torch
-    from bullseye_amd import kernels, synth
+    from bullseye_amd import synth
     from bullseye_amd.autograd import FacetImager, chi_squared
-    from bullseye_amd.binding import ImagingSpec
     from bullseye_amd.library_loader import PATH_FP32_KAHAN, PATH_FP64
     cfg = synth.CONFIGS["c2"]
     dev = torch.device("cuda:0")
     t0 = time.time()
     obs = synth.simulate(**cfg["sim"], seed=args.seed, device=dev)
-    rng = np.random.default_rng(args.seed + 7)
-    n_rows = 1_000_000 // obs.n_chan
-    keep = np.sort(rng.choice(obs.rows, n_rows, replace=False))
-    obs = synth.select_rows(obs, keep)
-    npix, cell, planes, S, OS = 2048, 1.5, 32, 31, 8
-    wl = np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :])
-    wmax = float(np.float32(wl.max()))
-    halves = kernels.linear_plane_supports(planes, 3, S)
-    conv = kernels.wproj_kernels(S, OS, planes, wmax, npix, cell, half_supports=halves,
-                                 dtype=np.complex64)
-    ra0, dec0 = obs.phase_centre
-    centre = np.array([[ra0 + 0.2 * 3600.0 / np.cos(np.radians(dec0 / 3600.0)), dec0]])
-    spec = ImagingSpec(npix=npix, cell=cell, conv=conv, support=S, oversample=OS,
-                       facet_centres=centre, wplanes=planes, wmax=wmax)
-    shape = (1, 1, 1, npix, npix)
-    img = np.zeros(npix * npix)
-    img[rng.choice(img.size, 500, replace=False)] = rng.uniform(0, 1, 500)
-    img = img.reshape(shape)
+    obs, spec, halves, img = synth.differentiable_facet(obs, args.seed + 7, device=dev)
     log("synthesised %d rows x %d ch (%d samples), kernels in %.1fs" % (
         obs.rows, obs.n_chan, obs.rows * obs.n_chan, time.time() - t0))
     calibration_copy(args.calibration_copy_mib)
@@ -760,7 +742,7 @@ def batch_workload(args, steps, warmup, compare_fp64, compare_tiles, cpu):
     import numpy as np
     import torch
     from bullseye_amd import kernels, synth
-    from bullseye_amd.batch import BatchFacet, FacetBatch
+    from bullseye_amd.batch import FacetBatch
     from bullseye_amd.binding import Binding, ImagingSpec
     from bullseye_amd.library_loader import (BATCH_PER_FACET, BATCH_PERSISTENT_GRAPH,
                                              PATH_FP32_KAHAN, PATH_FP64, bullseye_options,
@@ -772,23 +754,9 @@ def batch_workload(args, steps, warmup, compare_fp64, compare_tiles, cpu):
     sim = dict(cfg["sim"])
     sim["n_time"] = args.batch_timesteps
     obs = synth.simulate(**sim, seed=args.seed, device=dev)
-    wmax = float(np.float32(np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :]).max()))
-    plane_counts = [16, 24, 32, 48, 64, 96, 128]
-    ksets = []
-    for P in plane_counts:
-        halves = kernels.linear_plane_supports(P, 3, im["support"])
-        conv = kernels.wproj_kernels(im["support"], im["oversample"], P, wmax, im["npix"],
-                                     im["cell"], half_supports=halves, dtype=np.complex64)
-        ksets.append((conv, halves))
-    rng = np.random.default_rng(args.seed + 200)
     nf = args.batch_facets
-    sizes = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), nf)) / 2) * 2).astype(int)
-    kid = rng.integers(0, len(plane_counts), nf)
-    psf = np.zeros(nf, bool)
-    psf[rng.choice(nf, nf // 4, replace=False)] = True
-    cen = synth.facet_centres(nf, 2.0, obs.phase_centre, rng)
-    facets = [BatchFacet(int(sizes[i]), im["cell"], (float(cen[i, 0]), float(cen[i, 1])),
-                         int(kid[i]), bool(psf[i])) for i in range(nf)]
+    facets, ksets, plane_counts, wmax, sizes, psf = synth.batched_facets(
+        obs, nf, args.seed + 200, device=dev)
     log("synthesised %d rows x %d ch, %d kernel stacks, %d facets (%.0f Mpix) in %.1fs" % (
         obs.rows, obs.n_chan, len(ksets), nf, float((sizes.astype(float) ** 2).sum()) / 1e6,
         time.time() - t0))

@@ -78,7 +78,7 @@ def linear_plane_supports(wplanes, s_min, s_max):
 
 
 def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_support=3,
-                  half_supports=None, dtype=np.complex128):
+                  half_supports=None, dtype=np.complex128, device=None):
     """2D w-projection kernels in the reference's layout complex[wplanes][csz][csz].
 
     Plane p is the kernel for w_p = p * wmax / (wplanes - 1) (the gridder picks
@@ -86,7 +86,8 @@ def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_suppor
     of T(l) T(m) exp(-2 pi i w_p (n - 1)) sampled on the image grid that the oversampled kernel's
     DFT spans (OS times the field of view), where T is the image-plane response of the PSWF AA
     kernel of half support aa_support.  Planes are normalised so plane 0 sums to one, then cut
-    to their half supports (kernels.truncate_wkernels) when half_supports is given.
+    to their half supports (kernels.truncate_wkernels) when half_supports is given.  With a torch
+    device the transforms run there (float64; equal to the NumPy result to rounding).
     """
     csz = aa_kernel_size(support, oversample)
     fov = nx * cell_arcsec * np.pi / 648000.0
@@ -100,11 +101,21 @@ def wproj_kernels(support, oversample, wplanes, wmax, nx, cell_arcsec, aa_suppor
     taper2 = np.outer(taper1, taper1)
     lg, mg = np.meshgrid(lm, lm, indexing="ij")
     nm1 = np.sqrt(np.maximum(1.0 - lg ** 2 - mg ** 2, 0.0)) - 1.0
-    out = np.empty((wplanes, csz, csz), dtype=np.complex128)
     step = wmax / float(max(wplanes - 1, 1))
-    for p in range(wplanes):
-        img = taper2 * np.exp(-2j * np.pi * (p * step) * nm1)
-        out[p] = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(img)))
+    if device is not None:
+        import torch
+        t2 = torch.as_tensor(taper2, device=device)
+        nm = torch.as_tensor(nm1, device=device)
+        planes = []
+        for p in range(wplanes):
+            img = t2 * torch.exp(-2j * np.pi * (p * step) * nm)
+            planes.append(torch.fft.fftshift(torch.fft.ifft2(torch.fft.ifftshift(img))).cpu())
+        out = torch.stack(planes).numpy()
+    else:
+        out = np.empty((wplanes, csz, csz), dtype=np.complex128)
+        for p in range(wplanes):
+            img = taper2 * np.exp(-2j * np.pi * (p * step) * nm1)
+            out[p] = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(img)))
     out /= out[0].real.sum()
     if half_supports is not None:
         out = truncate_wkernels(out, half_supports, support, oversample)

@@ -217,3 +217,59 @@ CONFIGS = {
                image=dict(npix=4096, cell=1.5, support=31, oversample=8, facets=0,
                           wplanes=128)),
 }
+
+
+def batched_facets(obs, n_facets, seed, device=None):
+    """The batched uneven-facet workload over an observation of the w-projection workload:
+    n_facets facets with sizes log-uniform over 128^2-2048^2 (even) at 1.5", centres within 2 deg
+    of the phase centre, each with one of seven w-kernel stacks (16, 24, 32, 48, 64, 96 or 128
+    planes, half supports 3 -> 31 linear in |w|, OS x8, complex64) and a quarter of them
+    sampling-function facets.  Returns (facets [bullseye_amd.batch.BatchFacet], kernel sets
+    [(conv, half_supports)], plane counts, wmax, sizes, psf flags)."""
+    from . import kernels
+    from .batch import BatchFacet
+    im = CONFIGS["c2"]["image"]
+    wmax = float(np.float32(np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :]).max()))
+    plane_counts = [16, 24, 32, 48, 64, 96, 128]
+    ksets = []
+    for P in plane_counts:
+        halves = kernels.linear_plane_supports(P, 3, im["support"])
+        conv = kernels.wproj_kernels(im["support"], im["oversample"], P, wmax, im["npix"],
+                                     im["cell"], half_supports=halves, dtype=np.complex64,
+                                     device=device)
+        ksets.append((conv, halves))
+    rng = np.random.default_rng(seed)
+    nf = n_facets
+    sizes = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), nf)) / 2) * 2).astype(int)
+    kid = rng.integers(0, len(plane_counts), nf)
+    psf = np.zeros(nf, bool)
+    psf[rng.choice(nf, nf // 4, replace=False)] = True
+    cen = facet_centres(nf, 2.0, obs.phase_centre, rng)
+    facets = [BatchFacet(int(sizes[i]), im["cell"], (float(cen[i, 0]), float(cen[i, 1])),
+                         int(kid[i]), bool(psf[i])) for i in range(nf)]
+    return facets, ksets, plane_counts, wmax, sizes, psf
+
+
+def differentiable_facet(obs, seed, device=None, n_samples=1_000_000, n_model=500):
+    """The differentiable-imaging workload over an observation of the w-projection workload: a
+    seeded subset of rows holding n_samples samples (all channels), one 2048^2 facet at 1.5"
+    centred 0.2 deg from the phase centre, 32 w-planes (half supports 3 -> 31 linear in |w|, OS
+    x8, complex64) and a sparse real model image of n_model pixels ~ U(0, 1).  Returns
+    (observation subset, ImagingSpec, half supports, model image [1, 1, 1, 2048, 2048])."""
+    from . import kernels
+    from .binding import ImagingSpec
+    rng = np.random.default_rng(seed)
+    keep = np.sort(rng.choice(obs.rows, n_samples // obs.n_chan, replace=False))
+    sub = select_rows(obs, keep)
+    npix, cell, planes, S, OS = 2048, 1.5, 32, 31, 8
+    wmax = float(np.float32(np.abs(sub.uvw[:, 2:3] / sub.wavelengths[0][None, :]).max()))
+    halves = kernels.linear_plane_supports(planes, 3, S)
+    conv = kernels.wproj_kernels(S, OS, planes, wmax, npix, cell, half_supports=halves,
+                                 dtype=np.complex64, device=device)
+    ra0, dec0 = sub.phase_centre
+    centre = np.array([[ra0 + 0.2 * 3600.0 / np.cos(np.radians(dec0 / 3600.0)), dec0]])
+    spec = ImagingSpec(npix=npix, cell=cell, conv=conv, support=S, oversample=OS,
+                       facet_centres=centre, wplanes=planes, wmax=wmax)
+    img = np.zeros(npix * npix)
+    img[rng.choice(img.size, n_model, replace=False)] = rng.uniform(0, 1, n_model)
+    return sub, spec, halves, img.reshape(1, 1, 1, npix, npix)

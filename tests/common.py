@@ -149,3 +149,27 @@ def wproj_spec(obs, mode, precision, S=7, OS=4, planes=8, halves=None, seed=0, n
     w = np.abs(obs.uvw[:, 2:3] / obs.wavelengths[0][None, :])
     spec.wmax = float(np.float32(w.max() * 0.9))
     return spec, halves
+
+
+def oracle_chi2_gradient(obs, spec, mode, image, data):
+    """Float64 gradient of sum |G^T F(I) - V|^2 with respect to the real image I, from the oracle
+    alone: r = G^T F(I) - V by oracle_degrid, G r by the oracle gridder, and the FFT chain in
+    NumPy (F is the real transpose of finalize's chain).  Returns (gradient, loss)."""
+    from oracle import oracle
+    ref = Binding("double", obs, spec)
+    grid = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(image, axes=(-2, -1))), axes=(-2, -1))
+    vm = oracle.degrid(ref, mode, grid)
+    del ref
+    r = vm - data
+    saved = obs.vis
+    obs.vis = r
+    try:
+        rb = Binding("double", obs, spec)
+        oracle.run(rb, mode, normalize=False)
+    finally:
+        obs.vis = saved
+    gr = rb.output
+    n = gr.shape[-1] * gr.shape[-2]
+    img = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(gr, axes=(-2, -1))) * n, axes=(-2, -1))
+    loss = float((np.abs(r) ** 2).sum())
+    return 2 * img.real, loss

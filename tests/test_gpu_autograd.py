@@ -15,11 +15,10 @@ import numpy as np
 import pytest
 import torch
 
-from common import make_spec, round_to, small_obs, wproj_spec
+from common import (make_spec, oracle_chi2_gradient as reference_gradient, round_to, small_obs,
+                    wproj_spec)
 from bullseye_amd.autograd import DegridFunction, FacetImager, GridFunction, chi_squared
-from bullseye_amd.binding import Binding
 from bullseye_amd.library_loader import PATH_FP32_KAHAN, PATH_FP64
-from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -43,27 +42,6 @@ def sparse_image(shape, n, seed):
     idx = rng.choice(flat.size, n, replace=False)
     flat[idx] = rng.uniform(0, 1, n)
     return img
-
-
-def reference_gradient(obs, spec, mode, image, data):
-    """Float64 gradient of sum |G^T F(I) - V|^2 from the oracle and NumPy."""
-    ref = Binding("double", obs, spec)
-    grid = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(image, axes=(-2, -1))), axes=(-2, -1))
-    vm = oracle.degrid(ref, mode, grid)
-    r = vm - data
-    obs_r = obs
-    saved = obs_r.vis
-    obs_r.vis = r
-    try:
-        rb = Binding("double", obs_r, spec)
-        oracle.run(rb, mode, normalize=False)
-    finally:
-        obs_r.vis = saved
-    gr = rb.output
-    n = gr.shape[-1] * gr.shape[-2]
-    img = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(gr, axes=(-2, -1))) * n, axes=(-2, -1))
-    loss = float((np.abs(r) ** 2).sum())
-    return 2 * img.real, loss
 
 
 @pytest.mark.parametrize("kernel", ["aa", "wproj"])
