@@ -4,10 +4,13 @@ grid at 2", a 7x7 PSWF oversampled x63, no w-projection -- the shape of the refe
 benchmark (bullseye/mo/benchmarks/benchmark.cpp:97-289, one grid_single_pol facet).  The oracle
 grids all 7.7 M samples (3.6e8 taps) on one core in seconds.
 
-* FP64 path of the DOUBLE build: rms <= 1e-11 and peak <= 1e-9 of the oracle's peak, and the
-  normalisation sum to 1e-11.  The oracle runs the AA path under FE_DOWNWARD
-  (cpu_algorithm/convolution_policies.h:73-75) and the device rounds its coordinate products to
-  nearest; this bar shows that no sample of the workload changes its taps over that difference.
+* FP64 path of the DOUBLE build: rms <= 1e-11 and peak <= 1e-9 of the oracle's peak.  The
+  oracle runs the AA path under FE_DOWNWARD (cpu_algorithm/convolution_policies.h:73-75) and the
+  device rounds its coordinate products to nearest; this bar shows that no sample of the
+  workload changes its taps over that difference.
+* The normalisation sum to 2e-9 relative: the oracle adds its 7.4 M per-sample terms in one
+  running sum, every addition rounded towards -inf, which biases the total by up to one ulp per
+  term (~4e-10 of it here); the device sums in another order, rounded to nearest.
 * FP32-compensated path of the MIXED build: rms <= 1e-6 of the peak on the grid, and <= 1e-5
   RMS on the image after the same float64 shift / FFT on both sides.
 """
@@ -77,7 +80,7 @@ def test_fp64_path_matches_oracle(plain):
     rms, peak = rel_err(got, want)
     print("rms %.3e peak %.3e" % (rms, peak))
     assert rms <= 1e-11 and peak <= 1e-9, (rms, peak)
-    np.testing.assert_allclose(norm, norm_want, rtol=1e-11)
+    np.testing.assert_allclose(norm, norm_want, rtol=2e-9)
 
 
 def test_fp32_compensated_matches_oracle(plain):
@@ -89,4 +92,4 @@ def test_fp32_compensated_matches_oracle(plain):
     print("grid rms %.3e peak %.3e; image rms %.3e peak %.3e" % (rms, peak, irms, ipeak))
     assert rms <= 1e-6 and peak <= 2e-5, (rms, peak)
     assert irms <= 1e-5, irms
-    np.testing.assert_allclose(norm, norm_want, rtol=1e-9)
+    np.testing.assert_allclose(norm, norm_want, rtol=2e-9)
