@@ -74,8 +74,23 @@ struct State {
     double grid_seconds = 0, inv_seconds = 0;
     bullseye_stats stats{};
     unsigned long long last_counters[2] = {0, 0};
+    int device = 0;                 // the HIP device of the context (initLibrary's current one)
+    // inside bullseye_grid_noalloc: the call's tables are carved from this arena
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
 static State g;
+
+// Scratch of the allocation-free gridding entry point (bullseye_grid_noalloc): the per-call
+// tables of one gridding call -- inverted Jones terms, facet geometry, the baseline sort's keys,
+// permutation, counts and workspace, the walk-order measurement, the row and channel tables --
+// carved out of one static device array.  Every GPU the library runs on holds its own copy
+// until the process ends.  Host code never uses the array's name as an address: initLibrary
+// resolves each device's copy with hipGetSymbolAddress and keeps the address per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
 
 int fail(const char *fmt, ...) {
     char buf[512];
@@ -89,6 +104,23 @@ int fail(const char *fmt, ...) {
 
 
 static bool dev_mode() { return g.opts.device_resident != 0; }
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Device memory for one of a gridding call's tables: carved from the scratch arena inside
+// bullseye_grid_noalloc, otherwise the library buffer b, grown as needed.
+static hipError_t table(DevBuf &b, size_t bytes, void **out) {
+    if (g.arena) {
+        const size_t off = align256(g.arena_used);
+        if (off > g.arena_cap || bytes > g.arena_cap - off) return hipErrorOutOfMemory;
+        *out = g.arena + off;
+        g.arena_used = off + bytes;
+        return hipSuccess;
+    }
+    hipError_t e = b.ensure(bytes);
+    *out = b.p;
+    return e;
+}
 
 // Copy `bytes` from the caller's array to a device staging buffer (host mode) or use it as
 // is (device-resident mode).
@@ -320,9 +352,10 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
         if (!a.time_ids) return fail("timestamp_ids is NULL");
         // jones_2x2.h:113-126 invert_all, into a library buffer in the compute precision
         const bool fp64 = g.path == PATH_FP64;
-        HIP_TRY(g.jinv.ensure(nj * 8 * (fp64 ? 8 : 4)));
-        HIP_TRY(invert_jones(a.jones, g.jinv.p, nj, fp64, g.errflag.as<int>(), g.stream));
-        a.jones_inv = g.jinv.p;
+        void *jinv = nullptr;
+        HIP_TRY(table(g.jinv, nj * 8 * (fp64 ? 8 : 4), &jinv));
+        HIP_TRY(invert_jones(a.jones, jinv, nj, fp64, g.errflag.as<int>(), g.stream));
+        a.jones_inv = jinv;
     }
 
     // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre)
@@ -349,10 +382,11 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
             fg.dn = 1 - sqrt(1 - fg.dl * fg.dl - fg.dm * fg.dm);
         }
     }
-    HIP_TRY(g.geom.ensure(g.F * sizeof(FacetGeom)));
-    HIP_TRY(hipMemcpyAsync(g.geom.p, geo.data(), g.F * sizeof(FacetGeom), hipMemcpyHostToDevice,
+    void *geom = nullptr;
+    HIP_TRY(table(g.geom, g.F * sizeof(FacetGeom), &geom));
+    HIP_TRY(hipMemcpyAsync(geom, geo.data(), g.F * sizeof(FacetGeom), hipMemcpyHostToDevice,
                            g.stream));
-    a.geom = g.geom.as<FacetGeom>();
+    a.geom = reinterpret_cast<const FacetGeom *>(geom);
     a.conv = g.path == PATH_FP64 ? g.conv_d.p : g.conv_f.p;
     a.tapsum = g.tapsum.as<double>();
     a.error_flag = g.errflag.as<int>();
@@ -419,7 +453,32 @@ static int validate_call(const gridding_parameters *p, int mode) {
     return 0;
 }
 
-static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
+static size_t call_baselines(const gridding_parameters *p) {
+    return p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
+}
+
+// Bytes of the tables an AA gridding call carves (table()), each rounded to 256 bytes: an
+// upper bound of what bullseye_grid_noalloc needs from its scratch.
+static size_t aa_call_table_bytes(const gridding_parameters *p, int mode) {
+    const size_t rows = p->row_count, B = call_baselines(p);
+    const size_t nfc = std::max<size_t>(1, p->spw_count) * p->channel_count;
+    size_t n = align256(g.F * sizeof(FacetGeom));
+    if (mode == MODE_4J && rows > 0)
+        n += align256(p->no_timestamps_read * p->antenna_count * g.F *
+                      std::max<size_t>(1, p->spw_count) * p->channel_count * 8 *
+                      (g.path == PATH_FP64 ? 8 : 4));
+    n += 4 * align256(rows * 4) + 2 * align256((B + 1) * 4);
+    n += align256(ingest_workspace_bytes((uint32_t)rows, (uint32_t)B));
+    n += align256(walk_order_bytes());
+    n += align256(std::max<size_t>(1, rows) * sizeof(RowRec));
+    n += align256(std::max<size_t>(1, nfc) * sizeof(ChanRec));
+    return n;
+}
+
+// One gridding call.  noalloc (bullseye_grid_noalloc): the tables come from the scratch arena
+// and no timing events are created, so the call allocates nothing.
+static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
+                     bool noalloc = false) {
     if (!g.initialized) return fail("initLibrary has not been called");
     const bool psf = mode == MODE_PSF;
     const bool wproj = g.wplanes > 1;
@@ -427,34 +486,40 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     const int W = (int)(2 * p->conv_support + 1);
     const size_t rows = p->row_count, C = p->channel_count;
     const size_t nspw = std::max<size_t>(1, p->spw_count);
-    const size_t B = p->baseline_count ? p->baseline_count : p->antenna_count * (p->antenna_count + 1) / 2;
+    const size_t B = call_baselines(p);
 
     // registered before anything can fail, so the barrier releases the events of a failed call
-    g.pending.push_back(TimedCall{});
-    TimedCall &tc = g.pending.back();
-    HIP_TRY(hipEventCreate(&tc.start));
-    HIP_TRY(hipEventCreate(&tc.copied));
-    HIP_TRY(hipEventCreate(&tc.k0));
-    HIP_TRY(hipEventCreate(&tc.k1));
-    HIP_TRY(hipEventCreate(&tc.stop));
-    HIP_TRY(hipEventRecord(tc.start, g.stream));
+    TimedCall none{};
+    TimedCall *tcp = &none;
+    if (!noalloc) {
+        g.pending.push_back(TimedCall{});
+        tcp = &g.pending.back();
+        HIP_TRY(hipEventCreate(&tcp->start));
+        HIP_TRY(hipEventCreate(&tcp->copied));
+        HIP_TRY(hipEventCreate(&tcp->k0));
+        HIP_TRY(hipEventCreate(&tcp->k1));
+        HIP_TRY(hipEventCreate(&tcp->stop));
+        HIP_TRY(hipEventRecord(tcp->start, g.stream));
+    }
+    TimedCall &tc = *tcp;
 
     GridArgs a{};
     if (setup_inputs(p, mode, facet, phase, psf, true, a, tc.copied)) return 1;
 
     // ingest: stable sort of rows by baseline
-    HIP_TRY(g.perm.ensure(rows * 4));
-    HIP_TRY(g.keys.ensure(rows * 4));
-    HIP_TRY(g.vals.ensure(rows * 4));
-    HIP_TRY(g.keys_out.ensure(rows * 4));
-    HIP_TRY(g.counts.ensure((B + 1) * 4));
-    HIP_TRY(g.blstart.ensure((B + 1) * 4));
+    void *perm, *keys, *vals, *keys_out, *counts, *blstart, *ws;
+    HIP_TRY(table(g.perm, rows * 4, &perm));
+    HIP_TRY(table(g.keys, rows * 4, &keys));
+    HIP_TRY(table(g.vals, rows * 4, &vals));
+    HIP_TRY(table(g.keys_out, rows * 4, &keys_out));
+    HIP_TRY(table(g.counts, (B + 1) * 4, &counts));
+    HIP_TRY(table(g.blstart, (B + 1) * 4, &blstart));
     const size_t wsb = ingest_workspace_bytes((uint32_t)rows, (uint32_t)B);
-    HIP_TRY(g.ws.ensure(wsb));
+    HIP_TRY(table(g.ws, wsb, &ws));
     HIP_TRY(ingest_sort(a.ant1, a.ant2, (uint32_t)rows, (uint32_t)p->antenna_count, (uint32_t)B,
-                        g.keys.as<uint32_t>(), g.vals.as<uint32_t>(), g.keys_out.as<uint32_t>(),
-                        g.counts.as<uint32_t>(), g.perm.as<uint32_t>(), g.blstart.as<uint32_t>(),
-                        g.ws.p, g.ws.bytes, g.errflag.as<int>(), g.stream));
+                        (uint32_t *)keys, (uint32_t *)vals, (uint32_t *)keys_out,
+                        (uint32_t *)counts, (uint32_t *)perm, (uint32_t *)blstart, ws, wsb,
+                        g.errflag.as<int>(), g.stream));
 
     // do all enabled channels map to one grid slice? (then channels can share a stream)
     bool same_slice = true;
@@ -476,8 +541,8 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
                   ? (uint32_t)std::max<size_t>(1, std::min<size_t>(g.opts.channels_per_stream, C))
                   : same_slice ? (uint32_t)std::min<size_t>(C, 8) : 1;
 
-    a.perm = g.perm.as<uint32_t>();
-    a.bl_start = g.blstart.as<uint32_t>();
+    a.perm = (const uint32_t *)perm;
+    a.bl_start = (const uint32_t *)blstart;
     a.grid = psf ? g.psfgrid.as<double>() : g.grid.as<double>();
     a.norm = g.norm.as<double>();
     a.counters = g.counters.as<unsigned long long>();
@@ -488,27 +553,29 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
     a.n_chan_blocks = (uint32_t)((C + cps - 1) / cps);
     a.n_segments = nseg;
     a.n_streams = (uint64_t)B * a.n_chan_blocks * nseg * g.F;
-    HIP_TRY(g.walk.ensure(walk_order_bytes()));
-    a.time_inner = g.walk.as<int>();
-    HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, g.walk.as<int>(), g.stream));
+    void *walk;
+    HIP_TRY(table(g.walk, walk_order_bytes(), &walk));
+    a.time_inner = (const int *)walk;
+    HIP_TRY(launch_walk_order(a, (int)g.opts.walk_order, (int *)walk, g.stream));
 
     // sorted row table (every gridder's sample preparation) and channel table (the AA gridder)
     {
         const size_t nfc = nspw * C;
-        HIP_TRY(g.rowrec.ensure(std::max<size_t>(1, rows) * sizeof(RowRec)));
-        HIP_TRY(g.chanrec.ensure(std::max<size_t>(1, nfc) * sizeof(ChanRec)));
-        a.rowrec = g.rowrec.as<RowRec>();
-        a.chanrec = g.chanrec.as<ChanRec>();
+        void *rowrec, *chanrec;
+        HIP_TRY(table(g.rowrec, std::max<size_t>(1, rows) * sizeof(RowRec), &rowrec));
+        HIP_TRY(table(g.chanrec, std::max<size_t>(1, nfc) * sizeof(ChanRec), &chanrec));
+        a.rowrec = (const RowRec *)rowrec;
+        a.chanrec = (const ChanRec *)chanrec;
         a.chan_lds = nfc <= CHAN_LDS_MAX ? 1u : 0u;
         HIP_TRY(build_row_tables(a, (uint32_t)rows, mode == MODE_4J && rows > 0,
-                                 g.rowrec.as<RowRec>(), g.chanrec.as<ChanRec>(), g.stream));
+                                 (RowRec *)rowrec, (ChanRec *)chanrec, g.stream));
     }
 
     if (wproj) {
         // w-projection: streams are SEG-row segments bucketed by their support class
         const uint32_t SEG = 32;
         std::vector<uint32_t> bls(B + 1);
-        HIP_TRY(hipMemcpyAsync(bls.data(), g.blstart.p, (B + 1) * 4, hipMemcpyDeviceToHost,
+        HIP_TRY(hipMemcpyAsync(bls.data(), blstart, (B + 1) * 4, hipMemcpyDeviceToHost,
                                g.stream));
         HIP_TRY(hipStreamSynchronize(g.stream));
         uint32_t max_nb = 0;
@@ -560,16 +627,18 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase) {
                             ((uintptr_t)a.vis % 16) == 0 && ((uintptr_t)a.weights % 16) == 0 &&
                             ((uintptr_t)a.flags % 4) == 0) ? 1u : 0u;
         }
-        HIP_TRY(hipEventRecord(tc.k0, g.stream));
+        if (!noalloc) HIP_TRY(hipEventRecord(tc.k0, g.stream));
         if (specialised)
             HIP_TRY(launch_romein_w(a, mode, g.path, g.stream));
         else
             HIP_TRY(launch_romein_aa(a, mode, g.path, g.stream));
     }
-    HIP_TRY(hipEventRecord(tc.k1, g.stream));
-    tc.has_kernel = true;
-    HIP_TRY(hipEventRecord(tc.stop, g.stream));
-    tc.recorded = true;
+    if (!noalloc) {
+        HIP_TRY(hipEventRecord(tc.k1, g.stream));
+        tc.has_kernel = true;
+        HIP_TRY(hipEventRecord(tc.stop, g.stream));
+        tc.recorded = true;
+    }
     if (!dev_mode()) {
         // the caller may reuse its host arrays after gridding_barrier; staging copies are
         // complete once the stream reaches the kernel
@@ -762,6 +831,18 @@ static int init_impl(gridding_parameters *p) {
     if (2 * p->conv_support + 3 >= p->nx || 2 * p->conv_support + 3 >= p->ny)
         return fail("conv_support too large for the grid");
     if (!p->conv) return fail("conv is NULL");
+    HIP_TRY(hipGetDevice(&g.device));
+    if (g.device < 0 || g.device >= NOALLOC_MAX_DEVICES)
+        return fail("HIP device %d is beyond the library's %d", g.device, NOALLOC_MAX_DEVICES);
+    if (!g_scratch_ptr[g.device]) {
+        // this device's copy of the allocation-free call's scratch array, resolved once
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail("resolving the scratch array on device %d: %s", g.device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[g.device] = static_cast<unsigned char *>(ptr);
+    }
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     g.F = std::max<size_t>(1, p->num_facet_centres);
     g.cube = std::max<size_t>(1, p->cube_channel_dim_size);
@@ -792,6 +873,7 @@ static int init_impl(gridding_parameters *p) {
     HIP_TRY(hipMemset(g.errflag.p, 0, sizeof(int)));
     g.last_counters[0] = g.last_counters[1] = 0;
     g.host_norm.assign(g.F * g.cube * g.P, 0.0);
+    g.geo_host.assign(g.F, FacetGeom{});   // the per-call geometry reuses this host buffer
     p->normalization_terms = g.host_norm.data();
     HIP_TRY(fill_f64(g.norm.as<double>(), g.norm_stride * g.nslots, 0.0, g.stream));
     // seed the accumulation grid from the caller's buffer (gpu_algorithm/wrapper.cu:165-168)
@@ -1048,6 +1130,44 @@ int repack_input_data(gridding_parameters *p) {
     permute(p->field_array, 4);
     if (p->should_invert_jones_terms) permute(p->timestamp_ids, sizeof(size_t));
     return 0;
+}
+
+// No counterpart in the reference: grid_* / facet_* for callers that cannot allocate device
+// memory (HIP graph capture, serving loops).  AA gridding with device-resident buffers only;
+// the call's tables come from the library's static scratch array (NOALLOC_SCRATCH_BYTES per
+// device), the grids and normalisation sums from initLibrary.  Calls that overlap on one device
+// race on the scratch.  bullseye_stats is not updated by these calls.
+int bullseye_grid_noalloc(gridding_parameters *p, int mode) {
+    static const struct { int kmode; bool facet, phase; } modes[] = {
+        {MODE_1, false, false},   {MODE_1, true, true},    {MODE_2, false, false},
+        {MODE_2, true, true},     {MODE_4, false, false},  {MODE_4, true, true},
+        {MODE_4J, true, true},    {MODE_PSF, false, false}, {MODE_PSF, true, false}};
+    if (!g.initialized) return fail("initLibrary has not been called");
+    if (!p) return fail("bullseye_grid_noalloc: NULL params");
+    if (mode < 0 || mode > 8) return fail("bullseye_grid_noalloc: bad mode %d", mode);
+    if (!dev_mode())
+        return fail("bullseye_grid_noalloc needs device-resident buffers "
+                    "(bullseye_options.device_resident = 1)");
+    if (g.wplanes > 1)
+        return fail("bullseye_grid_noalloc grids with the AA kernel only (wplanes <= 1)");
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != g.device)
+        return fail("bullseye_grid_noalloc: device %d is current, the library runs on %d", cur,
+                    g.device);
+    const int km = modes[mode].kmode;
+    if (validate_call(p, km)) return 1;
+    const size_t need = aa_call_table_bytes(p, km);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail("bullseye_grid_noalloc: the call's tables need %zu bytes, the scratch holds "
+                    "%zu (grid fewer rows per call)", need, NOALLOC_SCRATCH_BYTES);
+    g.arena = g_scratch_ptr[g.device];
+    g.arena_cap = NOALLOC_SCRATCH_BYTES;
+    g.arena_used = 0;
+    const int rc = grid_call(p, km, modes[mode].facet, modes[mode].phase, true);
+    g.arena = nullptr;
+    g.arena_cap = g.arena_used = 0;
+    return rc;
 }
 
 int grid_single_pol(gridding_parameters *p) { return grid_call(p, MODE_1, false, false); }

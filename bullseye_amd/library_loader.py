@@ -25,7 +25,8 @@ EXTENSION_SYMBOLS = ("bullseye_last_error", "bullseye_set_options", "bullseye_ge
                      "bullseye_dft_predict", "bullseye_batch_kernel_set",
                      "bullseye_batch_set_tuning", "bullseye_batch_get_tuning",
                      "bullseye_batch_plan", "bullseye_batch_execute", "bullseye_batch_autotune",
-                     "bullseye_batch_release", "bullseye_fp64_atomic_rate")
+                     "bullseye_batch_release", "bullseye_fp64_atomic_rate",
+                     "bullseye_grid_noalloc")
 
 
 class bullseye_options(ctypes.Structure):
@@ -126,6 +127,7 @@ def load_library(architecture, precision, share_torch_runtime=True):
     lib.bullseye_fp64_atomic_rate.argtypes = [ctypes.POINTER(ctypes.c_double)]
     lib.bullseye_degrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p]
+    lib.bullseye_grid_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.bullseye_dft_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.bullseye_dft_predict.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                          ctypes.c_void_p]

@@ -256,6 +256,18 @@ enum bullseye_grid_mode {
     BULLSEYE_FACET_SAMPLING_FUNCTION = 8
 };
 
+/* Allocation-free gridding.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Grids one chunk like the entry point numbered `mode` (enum
+ * bullseye_grid_mode) with the AA kernel (wplanes <= 1) and device-resident buffers
+ * (bullseye_options.device_resident = 1), into the grids initLibrary set up.  The call's own
+ * tables (baseline sort, row and channel tables, facet geometry, inverted Jones terms) live in
+ * a static 64 MiB device array per GPU that the library holds until the process ends; a call
+ * whose tables exceed it is refused (600,000 rows per call fit without Jones terms).  It calls
+ * no hipMalloc, hipFree or allocating library routine.  Calls that overlap on one GPU race on
+ * that array; bullseye_stats is not updated. */
+int bullseye_grid_noalloc(gridding_parameters *params, int mode);
+
 /* Degridding: the adjoint of a gridding entry point.  No counterpart in bullseye's C++ library
  * (writing_and_wiki/notebooks/Resampling.ipynb sketches the gather in Python).  mode is one of
  * the visibility modes 0..6.  model has output_buffer's layout (complex [F][cube][P][ny][nx],
