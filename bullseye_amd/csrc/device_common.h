@@ -132,6 +132,18 @@ template <typename T> __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b)
     return {a.re + b.re, a.im + b.im};
 }
 template <typename T> __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return {a.re, -a.im}; }
+// a * conj(b)
+template <typename T> __device__ __forceinline__ cpx<T> cmul_conj(cpx<T> a, cpx<T> b) {
+    return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))};
+}
+// acc + a * b and acc + a * conj(b), chained by FMA
+template <typename T> __device__ __forceinline__ cpx<T> cmac(cpx<T> acc, cpx<T> a, cpx<T> b) {
+    return {fma(a.re, b.re, fma(-a.im, b.im, acc.re)), fma(a.re, b.im, fma(a.im, b.re, acc.im))};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cmac_conj(cpx<T> acc, cpx<T> a, cpx<T> b) {
+    return {fma(a.re, b.re, fma(a.im, b.im, acc.re)), fma(a.im, b.re, fma(-a.re, b.im, acc.im))};
+}
 
 // Kahan-compensated running sum of one real component.  The true sum is s - c.
 template <typename T, bool KAHAN> struct acc1 {

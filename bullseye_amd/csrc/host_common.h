@@ -62,6 +62,8 @@ hipError_t finalize_device(void *buf, uint32_t F, uint32_t n, uint32_t fstride, 
 hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t build_row_tables(const GridArgs &a, uint32_t rows, bool jones, RowRec *rowrec,
                             ChanRec *chanrec, hipStream_t st);
+hipError_t launch_facet_geom(const uvw_base_type *fc, double ra0, double dec0, uint32_t F,
+                             bool facet, bool phase, FacetGeom *out, hipStream_t st);
 
 // cpu_gpu_common/cu_common.h:63 -- a float constant in every precision build
 static const float ARCSEC_TO_RAD = (float)(M_PI / (180.0 * 3600.0));

@@ -223,6 +223,43 @@ hipError_t build_row_tables(const GridArgs &a, uint32_t rows, bool jones, RowRec
     return hipGetLastError();
 }
 
+// Per-facet geometry in FP64 on the device, from the facet centres (arcsec, as given):
+// baseline_transform_policies.h:120-143 (planar uvw transform) and phase_transform_policies.h:
+// 123-133 (phase offset); arcsec are converted with the reference's float constant
+// (cu_common.h:63).  grid_* modes use the phase centre: every term stays zero.
+__global__ void k_facet_geom(const uvw_base_type *fc, double ra0, double dec0, uint32_t F,
+                             int facet, int phase, FacetGeom *out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    constexpr double A2R = (double)(float)(3.14159265358979323846 / (180.0 * 3600.0));
+    FacetGeom g = {};
+    if (facet) {
+        const double ra = (double)fc[2 * f], dec = (double)fc[2 * f + 1];
+        const double d_ra = (ra - ra0) * A2R;
+        const double n_dec = dec * A2R, o_dec = dec0 * A2R;
+        const double li0 = cos(n_dec) * sin(d_ra);
+        const double mi0 = sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra);
+        const double ni0 = sin(n_dec) * sin(o_dec) + cos(n_dec) * cos(o_dec) * cos(d_ra);
+        g.u_term = li0 / ni0;
+        g.v_term = mi0 / ni0;
+        if (phase) {
+            g.dl = -cos(n_dec) * sin(d_ra);
+            g.dm = -(sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra));
+            g.dn = 1 - sqrt(1 - g.dl * g.dl - g.dm * g.dm);
+        }
+    }
+    out[f] = g;
+}
+
+hipError_t launch_facet_geom(const uvw_base_type *fc, double ra0, double dec0, uint32_t F,
+                             bool facet, bool phase, FacetGeom *out, hipStream_t st) {
+    if (!F) return hipSuccess;
+    if (facet && !fc) return hipErrorInvalidValue;
+    k_facet_geom<<<(F + 63) / 64, 64, 0, st>>>(fc, ra0, dec0, F, facet ? 1 : 0, phase ? 1 : 0,
+                                               out);
+    return hipGetLastError();
+}
+
 // A kernel, not hipMemsetAsync, so that it can be captured into a graph and replayed.
 hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
     if (!n) return hipSuccess;

@@ -2,9 +2,12 @@
 //
 // Same decomposition and semantics as romein.hip, with these changes for the common case:
 //  * W is a compile-time constant, so residues and ownership are constant-folded;
-//  * the lane that prepares a sample also writes the sample's taps *rotated to residue
-//    order* into its LDS record (cu[a] is the u-tap for the lane that owns x = a mod W, cv[j]
-//    the v-tap for the accumulator that owns y = j mod W), so the walk is loads + MACs;
+//  * the taps are staged in LDS *rotated to residue order*: row f of the table holds
+//    conv[f + (k mod W) * OS] for k in [0, 2W), so the W taps of a sample with fraction f and
+//    origin residue m, in residue order (tap j for the grid point congruent to j mod W), are
+//    the contiguous run starting at f * 2W + W - m.  The lane that prepares a sample writes the
+//    two run offsets into its LDS record; a walking lane reads its u-tap and the W v-taps from
+//    the table, so the walk is loads + MACs;
 //  * 4-correlation modes split the correlations over lanes (CPL = 1 per lane on the FP32
 //    paths, 2 on the FP64 path), which cuts the accumulator registers per lane;
 //  * FP32 compensated sums take one Kahan step per run of samples on an unchanged footprint:
@@ -33,13 +36,28 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, int NC, int W> struct Rec {
+// TR: the record carries the sample's residue-ordered taps (FP32 paths: the walk reads them with
+// the record); otherwise it carries the runs' offsets in the tap table (FP64 path: half the
+// record bytes, the taps read from the table by the walking lanes).
+template <typename T, int NC, int W, bool TR> struct Rec {
     int x0, y0, cgi, qf;    // footprint; qf (first record of a quad): 1 = the quad is uniform
     int qx0, qy0, qcgi, pad;  // the uniform quad's footprint (qcgi < 0: no valid sample)
     cpx<T> vis[NC];
     T cu[W];
     T cv[W];
 };
+template <typename T, int NC, int W> struct Rec<T, NC, W, false> {
+    int x0, y0, cgi, qf;
+    int qx0, qy0, qcgi, pad;
+    int tu, tv;             // first tap of the u and v runs in the rotated tap table
+    cpx<T> vis[NC];
+};
+
+// Entries of the rotated tap table for oversampling OS and width W: OS + 1 fraction rows of 2W
+// taps, then one row of zeros (the taps of a record that contributes nothing).
+__host__ __device__ constexpr uint32_t rot_table_len(uint32_t os, int w) {
+    return (os + 2) * 2u * (uint32_t)w;
+}
 
 template <int MODE, int PATH, int W> struct GridderW {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
@@ -51,7 +69,8 @@ template <int MODE, int PATH, int W> struct GridderW {
     static constexpr int GPW = 64 / G;                 // groups per wavefront
     typedef typename PathTraits<PATH>::acc_t T;
     static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
-    typedef Rec<T, NC, W> R_t;
+    static constexpr bool TR = PATH != PATH_FP64;
+    typedef Rec<T, NC, W, TR> R_t;
     // bytes between the record arrays of consecutive groups: 16-byte aligned and starting
     // 8 banks (of 4 bytes, 32 banks) after the previous group
     static constexpr int GRAW = (G * (int)sizeof(R_t) + 15) & ~15;
@@ -81,7 +100,7 @@ template <int MODE, int PATH, int W> struct GridderW {
     // the row arrays: spw range, enabled channel, grid slice range, row flag and field, then the
     // Jones ids.
     static __device__ __forceinline__ bool prep(const GridArgs &a, const FacetGeom *__restrict__ gp,
-                                                const T *sconv, const double *sts,
+                                                const T *stap, const double *sts,
                                                 const ChanRec *schan, uint32_t ri,
                                                 uint64_t jfacet, uint32_t c, R_t &r,
                                                 double nw[NC]) {
@@ -92,7 +111,7 @@ template <int MODE, int PATH, int W> struct GridderW {
             return false;
         }
         const uint32_t fc = s * a.chan + c;
-        const ChanRec ch = a.chan_lds ? schan[fc] : a.chanrec[fc];
+        const ChanRec ch = schan[fc];   // the channel table is staged in LDS
         if (!ch.enabled) return false;
         if (ch.cgi >= a.cube) {
             atomicOr(a.error_flag, 4);
@@ -153,16 +172,15 @@ template <int MODE, int PATH, int W> struct GridderW {
             }
         }
         if constexpr (MODE == MODE_4J) {  // V = P^-1 V (Q^-1)^H (correlation_gridding_policies.h:523-542)
-            const cpx<T> h0 = conjg(Q[0]), h1 = conjg(Q[2]), h2 = conjg(Q[1]), h3 = conjg(Q[3]);
-            const cpx<T> t0 = cadd(cmul(vis[0], h0), cmul(vis[1], h2));
-            const cpx<T> t1 = cadd(cmul(vis[0], h1), cmul(vis[1], h3));
-            const cpx<T> t2 = cadd(cmul(vis[2], h0), cmul(vis[3], h2));
-            const cpx<T> t3 = cadd(cmul(vis[2], h1), cmul(vis[3], h3));
-            const cpx<T> p0 = P[0], p1 = P[1], p2 = P[2], p3 = P[3];
-            vis[0] = cadd(cmul(p0, t0), cmul(p1, t2));
-            vis[1] = cadd(cmul(p0, t1), cmul(p1, t3));
-            vis[2] = cadd(cmul(p2, t0), cmul(p3, t2));
-            vis[3] = cadd(cmul(p2, t1), cmul(p3, t3));
+            // (Q^-1)^H = [conj q0, conj q2; conj q1, conj q3]; products chained by FMA
+            const cpx<T> t0 = cmac_conj(cmul_conj(vis[0], Q[0]), vis[1], Q[1]);
+            const cpx<T> t1 = cmac_conj(cmul_conj(vis[0], Q[2]), vis[1], Q[3]);
+            const cpx<T> t2 = cmac_conj(cmul_conj(vis[2], Q[0]), vis[3], Q[1]);
+            const cpx<T> t3 = cmac_conj(cmul_conj(vis[2], Q[2]), vis[3], Q[3]);
+            vis[0] = cmac(cmul(P[0], t0), P[1], t2);
+            vis[1] = cmac(cmul(P[0], t1), P[1], t3);
+            vis[2] = cmac(cmul(P[2], t0), P[3], t2);
+            vis[3] = cmac(cmul(P[2], t1), P[3], t3);
         }
         bool any = false;
 #pragma unroll
@@ -206,19 +224,20 @@ template <int MODE, int PATH, int W> struct GridderW {
         r.x0 = x0;
         r.y0 = y0;
         r.cgi = (int)ch.cgi;
-        const int xm = x0 % W, ym = y0 % W;
 #pragma unroll
         for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
-        // tap for residue j: sconv[f + mod_w(j - m) * OS] = sconv[f - m*OS + j*OS (+ W*OS if j < m)],
-        // indexes formed by additions (no per-tap multiply)
-        const int os = (int)a.OS, wos = W * os;
-        const int bu = fu - xm * os, bv = fv - ym * os;
-        int jos = 0;
+        // residue-ordered tap runs: tap j of a run is conv[f + ((j - m) mod W) * OS]
+        const int tu = fu * (2 * W) + W - x0 % W;
+        const int tv = fv * (2 * W) + W - y0 % W;
+        if constexpr (TR) {
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            r.cu[j] = sconv[bu + jos + (j < xm ? wos : 0)];
-            r.cv[j] = sconv[bv + jos + (j < ym ? wos : 0)];
-            jos += os;
+            for (int j = 0; j < W; ++j) {
+                r.cu[j] = stap[tu + j];
+                r.cv[j] = stap[tv + j];
+            }
+        } else {
+            r.tu = tu;
+            r.tv = tv;
         }
         const double ts = sts[fu] * sts[fv];
 #pragma unroll
@@ -282,9 +301,24 @@ template <int MODE, int PATH, int W> struct GridderW {
         pcgi = cgi;
     }
 
+    static __device__ __forceinline__ const T *v_taps(const R_t &rec, const T *tap) {
+        if constexpr (TR)
+            return rec.cv;
+        else
+            return tap + rec.tv;
+    }
+    static __device__ __forceinline__ T u_tap(const R_t &rec, const T *tap, int ua) {
+        if constexpr (TR)
+            return rec.cu[ua];
+        else
+            return tap[rec.tu + ua];
+    }
+
     // One sample into the register sums (classic Kahan step on the compensated path).
-    static __device__ __forceinline__ void add_one(const R_t &rec, int ua, int c0, Acc &acc) {
-        const T cu = rec.cu[ua];
+    static __device__ __forceinline__ void add_one(const R_t &rec, const T *tap, int ua, int c0,
+                                                   Acc &acc) {
+        const T cu = u_tap(rec, tap, ua);
+        const T *cvp = v_taps(rec, tap);
         cpx<T> vs[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
@@ -294,7 +328,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
 #pragma unroll
         for (int r = 0; r < W; ++r) {
-            const T cv = rec.cv[r];
+            const T cv = cvp[r];
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
                 if constexpr (KAHAN) {
@@ -332,8 +366,10 @@ template <int MODE, int PATH, int W> struct GridderW {
     }
 
     // One sample on the run's footprint: its products chained by FMA onto the run.
-    static __device__ __forceinline__ void run_one(const R_t &rec, int ua, int c0, Run &y) {
-        const T cu = rec.cu[ua];
+    static __device__ __forceinline__ void run_one(const R_t &rec, const T *tap, int ua, int c0,
+                                                   Run &y) {
+        const T cu = u_tap(rec, tap, ua);
+        const T *cvp = v_taps(rec, tap);
         cpx<T> vs[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
@@ -343,7 +379,7 @@ template <int MODE, int PATH, int W> struct GridderW {
         }
 #pragma unroll
         for (int r = 0; r < W; ++r) {
-            const T cv = rec.cv[r];
+            const T cv = cvp[r];
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
                 y[r][k][0] = fma(vs[k].re, cv, y[r][k][0]);
@@ -353,26 +389,27 @@ template <int MODE, int PATH, int W> struct GridderW {
     }
 
     // QUAD samples on the run's footprint.
-    static __device__ __forceinline__ void run_quad(const R_t *recs, int ua, int c0, Run &y) {
+    static __device__ __forceinline__ void run_quad(const R_t *recs, const T *tap, int ua, int c0,
+                                                    Run &y) {
 #pragma unroll
-        for (int q = 0; q < QUAD; ++q) run_one(recs[q], ua, c0, y);
+        for (int q = 0; q < QUAD; ++q) run_one(recs[q], tap, ua, c0, y);
     }
 
     // Publication of the lanes' partial sums needs every lane of the wave: walk() may return
     // early for lanes without a stream, run() then publishes for all of them.
-    static __device__ void run(const GridArgs &a, const T *sconv, const double *sts,
+    static __device__ void run(const GridArgs &a, const T *stap, const double *sts,
                                const ChanRec *schan, unsigned char *swave) {
         int nkey = -1;
         double nacc[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) nacc[k] = 0;
         unsigned long long cnt = 0;
-        walk(a, sconv, sts, schan, swave, nkey, nacc, cnt);
+        walk(a, stap, sts, schan, swave, nkey, nacc, cnt);
         if constexpr (MODE != MODE_PSF) publish_norm<NC>(a, nkey, nacc);
         publish_counts(a, cnt, cnt * (unsigned long long)(W * W));
     }
 
-    static __device__ __forceinline__ void walk(const GridArgs &a, const T *sconv,
+    static __device__ __forceinline__ void walk(const GridArgs &a, const T *stap,
                                                 const double *sts, const ChanRec *schan,
                                                 unsigned char *swave,
                                                 int &nkey,
@@ -406,6 +443,7 @@ template <int MODE, int PATH, int W> struct GridderW {
 
         const int ua = gl % W;          // u residue of this lane
         const int c0 = (gl / W) * CPL;  // first correlation of this lane
+        const uint32_t zero_run = (a.OS + 1) * 2u * W;   // the table's row of zeros
         R_t *grp = reinterpret_cast<R_t *>(swave + gw * GSTRIDE);
 
         Acc acc;
@@ -432,7 +470,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                     stream_sample(s, nrows, ncb, tin, tr, j);
                     const uint32_t c = c_lo + j;
                     double nw[NC];
-                    if (prep(a, gp, sconv, sts, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
+                    if (prep(a, gp, stap, sts, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
                         ++cnt;
                         if constexpr (MODE != MODE_PSF) {
                             if (rec.cgi != ncgi) {
@@ -450,10 +488,14 @@ template <int MODE, int PATH, int W> struct GridderW {
                     // contributes nothing; zero taps so it can ride inside a QUAD run
 #pragma unroll
                     for (int k = 0; k < NC; ++k) rec.vis[k] = {T(0), T(0)};
+                    if constexpr (TR) {
 #pragma unroll
-                    for (int j = 0; j < W; ++j) {
-                        rec.cu[j] = T(0);
-                        rec.cv[j] = T(0);
+                        for (int j = 0; j < W; ++j) {
+                            rec.cu[j] = T(0);
+                            rec.cv[j] = T(0);
+                        }
+                    } else {
+                        rec.tu = rec.tv = (int)zero_run;
                     }
                 }
                 grp[gl] = rec;
@@ -503,7 +545,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                             open_run(acc, y);
                             open = true;
                         }
-                        run_quad(grp + q * QUAD, ua, c0, y);
+                        run_quad(grp + q * QUAD, stap, ua, c0, y);
                     } else {
                         // a mixed quad: a sample on the open run's footprint extends the run;
                         // one elsewhere closes it, moves the ownership and opens a new run
@@ -518,7 +560,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                                 open_run(acc, y);
                                 open = true;
                             }
-                            run_one(rr, ua, c0, y);
+                            run_one(rr, stap, ua, c0, y);
                         }
                     }
                 }
@@ -529,7 +571,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                 const int cg = grp[kk].cgi;
                 if (cg < 0) continue;
                 move_to(a, facet, ua, c0, grp[kk].x0, grp[kk].y0, cg, px0, py0, pcgi, acc);
-                add_one(grp[kk], ua, c0, acc);
+                add_one(grp[kk], stap, ua, c0, acc);
             }
             wave_sync();
         }
@@ -551,20 +593,25 @@ __global__ __launch_bounds__(BLOCK) void romein_w_kernel(GridArgs a) {
     typedef GridderW<MODE, PATH, W> Gd;
     typedef typename Gd::T T;
     extern __shared__ __align__(16) unsigned char smem[];
-    T *sconv = reinterpret_cast<T *>(smem);
-    const uint32_t conv_bytes = (a.csz * sizeof(T) + 15) & ~15u;
+    // rotated tap table: row f < OS + 1 holds conv[f + (k mod W) * OS], k < 2W; row OS + 1 zeros
+    T *stap = reinterpret_cast<T *>(smem);
+    const uint32_t ntap = rot_table_len(a.OS, W);
+    const uint32_t conv_bytes = (ntap * sizeof(T) + 15) & ~15u;
     const T *gconv = reinterpret_cast<const T *>(a.conv);
-    for (uint32_t i = threadIdx.x; i < a.csz; i += BLOCK) sconv[i] = gconv[i];
+    for (uint32_t i = threadIdx.x; i < ntap; i += BLOCK) {
+        const uint32_t f = i / (2 * W), k = i - f * (2 * W);
+        stap[i] = f <= a.OS ? gconv[f + (k < W ? k : k - W) * a.OS] : T(0);
+    }
     // tap sums per fraction (OS + 1 doubles), then the channel table when it is staged
     double *sts = reinterpret_cast<double *>(smem + conv_bytes);
     const uint32_t ts_bytes = ((a.OS + 1) * 8 + 15) & ~15u;
     for (uint32_t i = threadIdx.x; i <= a.OS; i += BLOCK) sts[i] = a.tapsum[i];
     ChanRec *schan = reinterpret_cast<ChanRec *>(smem + conv_bytes + ts_bytes);
-    const uint32_t nfc = a.chan_lds ? a.spw_count * a.chan : 0u;
+    const uint32_t nfc = a.spw_count * a.chan;   // <= CHAN_LDS_MAX (launch_romein_w checks)
     for (uint32_t i = threadIdx.x; i < nfc; i += BLOCK) schan[i] = a.chanrec[i];
     __syncthreads();
-    Gd::run(a, sconv, sts, schan,
-            smem + conv_bytes + ts_bytes + nfc * sizeof(ChanRec) +
+    Gd::run(a, stap, sts, schan,
+            smem + conv_bytes + ts_bytes + ((nfc * sizeof(ChanRec) + 15) & ~15u) +
                 (threadIdx.x >> 6) * Gd::WAVE_BYTES);
 }
 
@@ -573,9 +620,9 @@ template <int MODE, int PATH, int W> static hipError_t launch_t(const GridArgs &
     const uint64_t waves = (a.n_streams + Gd::GPW - 1) / Gd::GPW;
     const uint64_t blocks = (waves + BLOCK / 64 - 1) / (BLOCK / 64);
     if (blocks == 0) return hipSuccess;
-    const size_t smem = ((a.csz * sizeof(typename Gd::T) + 15) & ~15u) +
+    const size_t smem = ((rot_table_len(a.OS, W) * sizeof(typename Gd::T) + 15) & ~15u) +
                         (((a.OS + 1) * 8 + 15) & ~15u) +
-                        (a.chan_lds ? (size_t)a.spw_count * a.chan * sizeof(ChanRec) : 0) +
+                        (((size_t)a.spw_count * a.chan * sizeof(ChanRec) + 15) & ~(size_t)15) +
                         (BLOCK / 64) * (size_t)Gd::WAVE_BYTES;
     if (!a.rowrec || !a.chanrec) return hipErrorInvalidValue;
     romein_w_kernel<MODE, PATH, W><<<dim3((uint32_t)blocks), dim3(BLOCK), smem, st>>>(a);
@@ -604,6 +651,8 @@ template <int W> static hipError_t launch_m(const GridArgs &a, int mode, int pat
 
 }  // namespace rw
 
+hipError_t launch_romein_aa(const GridArgs &a, int mode, int path, hipStream_t st);
+
 // Lane groups per wavefront for a support width, or 0 when no specialisation exists.
 int romein_w_groups_per_wave(int W, int mode, int path) {
     const int split = (mode == MODE_4 || mode == MODE_4J) ? (path == PATH_FP64 ? 2 : 4) : 1;
@@ -614,6 +663,8 @@ int romein_w_groups_per_wave(int W, int mode, int path) {
 }
 
 hipError_t launch_romein_w(const GridArgs &a, int mode, int path, hipStream_t st) {
+    // channel tables too large for LDS go to the generic kernel
+    if (!a.chan_lds) return launch_romein_aa(a, mode, path, st);
     switch (a.W) {
         case 3: return rw::launch_m<3>(a, mode, path, st);
         case 5: return rw::launch_m<5>(a, mode, path, st);

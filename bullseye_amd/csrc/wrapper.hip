@@ -56,7 +56,7 @@ struct State {
     DevBuf grid, psfgrid, norm, counters, errflag, conv_f, conv_d, tapsum, geom, taper;
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws, rowrec, chanrec;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
-        in_jones, in_wl, in_enabled, in_cgi;
+        in_jones, in_wl, in_enabled, in_cgi, in_fc;
     DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp, dft_smp,
         dft_comp;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
@@ -68,7 +68,6 @@ struct State {
     std::vector<uint32_t> wlist_host;
     std::vector<normalization_base_type> host_norm;
     std::vector<double> taper_x, taper_y;
-    std::vector<FacetGeom> geo_host;  // kept alive for the asynchronous upload
     // timing
     std::vector<TimedCall> pending;
     double grid_seconds = 0, inv_seconds = 0;
@@ -358,34 +357,18 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
         a.jones_inv = jinv;
     }
 
-    // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre)
-    HIP_TRY(hipStreamSynchronize(g.stream));  // the previous call's upload of geo_host is done
-    std::vector<FacetGeom> &geo = g.geo_host;
-    geo.assign(g.F, FacetGeom{});
-    for (size_t f = 0; f < g.F; ++f) {
-        FacetGeom &fg = geo[f];
-        memset(&fg, 0, sizeof fg);
-        if (!facet) continue;
-        const double ra0 = p->phase_centre_ra, dec0 = p->phase_centre_dec;
-        const double ra = p->facet_centres[2 * f], dec = p->facet_centres[2 * f + 1];
-        const double d_ra = (ra - ra0) * (double)ARCSEC_TO_RAD;
-        const double n_dec = dec * (double)ARCSEC_TO_RAD, o_dec = dec0 * (double)ARCSEC_TO_RAD;
-        // baseline_transform_policies.h:120-143
-        const double li0 = cos(n_dec) * sin(d_ra);
-        const double mi0 = sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra);
-        const double ni0 = sin(n_dec) * sin(o_dec) + cos(n_dec) * cos(o_dec) * cos(d_ra);
-        fg.u_term = li0 / ni0;
-        fg.v_term = mi0 / ni0;
-        if (phase) {  // phase_transform_policies.h:123-133
-            fg.dl = -cos(n_dec) * sin(d_ra);
-            fg.dm = -(sin(n_dec) * cos(o_dec) - cos(n_dec) * sin(o_dec) * cos(d_ra));
-            fg.dn = 1 - sqrt(1 - fg.dl * fg.dl - fg.dm * fg.dm);
-        }
+    // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre), computed
+    // on the device from the staged facet centres: no host buffer outlives the call
+    const void *fc = nullptr;
+    if (facet) {
+        if (!p->facet_centres) return fail("facet_centres is NULL");
+        HIP_TRY(stage(g.in_fc, p->facet_centres, g.F * 2 * sizeof(uvw_base_type), &fc));
     }
     void *geom = nullptr;
     HIP_TRY(table(g.geom, g.F * sizeof(FacetGeom), &geom));
-    HIP_TRY(hipMemcpyAsync(geom, geo.data(), g.F * sizeof(FacetGeom), hipMemcpyHostToDevice,
-                           g.stream));
+    HIP_TRY(launch_facet_geom((const uvw_base_type *)fc, (double)p->phase_centre_ra,
+                              (double)p->phase_centre_dec, (uint32_t)g.F, facet, phase,
+                              (FacetGeom *)geom, g.stream));
     a.geom = reinterpret_cast<const FacetGeom *>(geom);
     a.conv = g.path == PATH_FP64 ? g.conv_d.p : g.conv_f.p;
     a.tapsum = g.tapsum.as<double>();
@@ -811,7 +794,7 @@ static void release_resources() {
                       &g.rowrec, &g.chanrec,
                       &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
                       &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
-                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.out_tmp,
+                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.in_fc, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
                       &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
@@ -873,7 +856,6 @@ static int init_impl(gridding_parameters *p) {
     HIP_TRY(hipMemset(g.errflag.p, 0, sizeof(int)));
     g.last_counters[0] = g.last_counters[1] = 0;
     g.host_norm.assign(g.F * g.cube * g.P, 0.0);
-    g.geo_host.assign(g.F, FacetGeom{});   // the per-call geometry reuses this host buffer
     p->normalization_terms = g.host_norm.data();
     HIP_TRY(fill_f64(g.norm.as<double>(), g.norm_stride * g.nslots, 0.0, g.stream));
     // seed the accumulation grid from the caller's buffer (gpu_algorithm/wrapper.cu:165-168)
