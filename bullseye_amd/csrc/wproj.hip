@@ -21,6 +21,8 @@
 // classes wider than 16 use 1024-lane workgroups with up to PAIRS residue pairs per lane.
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "wproj_args.h"
 
 namespace bam {
@@ -28,7 +30,7 @@ namespace bam {
 namespace wp {
 
 constexpr int SEG = 32;        // default rows per stream segment
-constexpr int REC = 64;        // samples prepared per stream per step
+constexpr int REC = 128;       // samples prepared per stream per step (at most the stream's lanes)
 
 template <typename T, int NC> struct WRec {
     int x0, y0, xm, ym;     // class-footprint origin and its residues mod M
@@ -213,6 +215,7 @@ template <int MODE, int PATH> struct WPrep {
 // from the kernel's own (uniform) arguments.
 template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
+    static constexpr int UNR = PAIRS == 1 ? 4 : 2;   // records whose tap loads issue together
     typedef typename PathTraits<PATH>::acc_t T;
     static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
     typedef WRec<T, NC> R_t;
@@ -250,7 +253,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
             if (threadIdx.x == 0) s_item = atomicAdd(w.queue, 1u);
             __syncthreads();
             const uint32_t item = s_item;
-            if (item >= w.n_items) break;
+            if (item >= list_items(w, w.streams_per_block)) break;
             run_item(w, item, smem);
         }
     }
@@ -262,7 +265,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         const int sl = threadIdx.x - slot * L;       // lane within the stream
         const bool in_slot = slot < (int)w.streams_per_block;
         const uint64_t li = item * w.streams_per_block + slot;
-        bool live = in_slot && li < w.n_list;
+        bool live = in_slot && li < list_len(w);
         uint32_t stream = 0, fidx = 0;
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
@@ -273,12 +276,11 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
             stream = e & 0xffffffu;
             if (stream == 0xffffffu) stream = 0, live = false;   // padding entry
             wfp = w.facet_args + fidx;
-        } else {
-            if (live) stream = w.stream_list[li];
         }
         const WArgs &wf = *wfp;
         const GridArgs &a = wf.g;
-        R_t *recs = reinterpret_cast<R_t *>(smem) + slot * REC;
+        const uint32_t nrec = min((uint32_t)REC, (uint32_t)L);   // records per step
+        R_t *recs = reinterpret_cast<R_t *>(smem) + slot * nrec;
         __shared__ uint32_t s_steps;
         if (threadIdx.x == 0) s_steps = 0;
         __syncthreads();
@@ -287,13 +289,18 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         uint64_t cls_base = 0;
         const bool tin = *a.time_inner != 0;
         if (live) {
-            uint64_t t = stream;
-            facet = (uint32_t)(t % a.facets);
-            t /= a.facets;
-            const uint32_t seg = (uint32_t)(t % w.nseg);
-            t /= w.nseg;
-            const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
-            const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
+            uint32_t seg, cb, bl;
+            if constexpr (BATCH) {
+                uint64_t t = stream;
+                facet = (uint32_t)(t % a.facets);
+                t /= a.facets;
+                seg = (uint32_t)(t % w.nseg);
+                t /= w.nseg;
+                cb = (uint32_t)(t % a.n_chan_blocks);
+                bl = (uint32_t)(t / a.n_chan_blocks);
+            } else {
+                single_stream(w, a, li, facet, bl, cb, seg);
+            }
             cls_base = ((uint64_t)bl * a.n_chan_blocks + cb) * w.nseg_base;
             b0 = a.bl_start[bl];
             const uint32_t nb = a.bl_start[bl + 1] - b0;
@@ -303,7 +310,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
             ncb = min(a.chan, c_lo + a.chan_per_stream) - c_lo;
             nrows = r_hi > r_lo ? r_hi - r_lo : 0;
             nsamp = nrows * ncb;
-            if (sl == 0) atomicMax(&s_steps, (nsamp + REC - 1) / REC);
+            if (sl == 0) atomicMax(&s_steps, (nsamp + nrec - 1) / nrec);
         }
         __syncthreads();
         const uint32_t steps = s_steps;
@@ -338,8 +345,8 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         const int kbase = BATCH ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         for (uint32_t st = 0; st < steps; ++st) {
-            const uint32_t step0 = st * REC;
-            if (live && sl < REC) {
+            const uint32_t step0 = st * nrec;
+            if (live && sl < (int)nrec) {
                 R_t rec;
                 rec.cgi = -1;
                 const uint32_t s = step0 + sl;
@@ -368,30 +375,52 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                 recs[sl] = rec;
             }
             __syncthreads();
-            const int kmax = live ? (int)min((uint32_t)REC, nsamp > step0 ? nsamp - step0 : 0u) : 0;
-            for (int kk = 0; kk < kmax; ++kk) {
-                const R_t &r = recs[kk];
-                if (r.cgi < 0) continue;
+            const int kmax = live ? (int)min(nrec, nsamp > step0 ? nsamp - step0 : 0u) : 0;
+            // records in chunks of UNR: the chunk's tap loads are issued together, so their
+            // latency overlaps instead of one load in flight per lane at a time
+            for (int kk = 0; kk < kmax; kk += UNR) {
+                cpx<T> tq[UNR][PAIRS];
 #pragma unroll
-                for (int p = 0; p < PAIRS; ++p) {
-                    if (!own[p]) continue;
-                    const int ku = mod_m(pa[p] - r.xm, M);
-                    const int kv = mod_m(pb[p] - r.ym, M);
-                    const int x = r.x0 + ku, y = r.y0 + kv;
-                    if (x != px[p] || y != py[p] || r.cgi != pc[p]) {
-                        if (pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
-                        px[p] = x;
-                        py[p] = y;
-                        pc[p] = r.cgi;
+                for (int u = 0; u < UNR; ++u) {
+                    const R_t &r = recs[min(kk + u, kmax - 1)];
+#pragma unroll
+                    for (int p = 0; p < PAIRS; ++p) {
+                        const int tu = mod_m(pa[p] - r.xm, M) - r.off;
+                        const int tv = mod_m(pb[p] - r.ym, M) - r.off;
+                        const bool ok = kk + u < kmax && r.cgi >= 0 && own[p] &&
+                                        (unsigned)tu < (unsigned)r.wp &&
+                                        (unsigned)tv < (unsigned)r.wp;
+                        tq[u][p] = ok ? tapbase[r.tb + (uint32_t)(tv * r.wp + tu)]
+                                      : cpx<T>{T(0), T(0)};
                     }
-                    const int tu = ku - r.off, tv = kv - r.off;
-                    if ((unsigned)tu >= (unsigned)r.wp || (unsigned)tv >= (unsigned)r.wp) continue;
-                    const cpx<T> tap = tapbase[r.tb + (uint32_t)(tv * r.wp + tu)];
+                }
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) {
-                        const cpx<T> m = cmul(r.vis[k], tap);
-                        acc[p][k][0].add(m.re);
-                        acc[p][k][1].add(m.im);
+                for (int u = 0; u < UNR; ++u) {
+                    if (kk + u >= kmax) break;
+                    const R_t &r = recs[kk + u];
+                    if (r.cgi < 0) continue;
+#pragma unroll
+                    for (int p = 0; p < PAIRS; ++p) {
+                        if (!own[p]) continue;
+                        const int ku = mod_m(pa[p] - r.xm, M);
+                        const int kv = mod_m(pb[p] - r.ym, M);
+                        const int x = r.x0 + ku, y = r.y0 + kv;
+                        if (x != px[p] || y != py[p] || r.cgi != pc[p]) {
+                            if (pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
+                            px[p] = x;
+                            py[p] = y;
+                            pc[p] = r.cgi;
+                        }
+                        const int tu = ku - r.off, tv = kv - r.off;
+                        if ((unsigned)tu >= (unsigned)r.wp || (unsigned)tv >= (unsigned)r.wp)
+                            continue;
+                        const cpx<T> tap = tq[u][p];
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) {
+                            const cpx<T> m = cmul(r.vis[k], tap);
+                            acc[p][k][0].add(m.re);
+                            acc[p][k][1].add(m.im);
+                        }
                     }
                 }
             }
@@ -574,7 +603,7 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
             uint32_t it = 0;
             if (lane == 0) it = atomicAdd(w.queue, 1u);
             it = __shfl(it, 0, 64);
-            if (it >= w.n_items) break;
+            if (it >= list_items(w, GPW)) break;
             item(w, it, swave);
         }
     }
@@ -599,27 +628,27 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
         const int gl = lane - gw * G;
         if (gw >= GPW) return;
         const uint64_t li = it * (uint64_t)GPW + gw;
-        if (li >= w.n_list) return;
-        uint32_t stream, fidx = 0;
+        if (li >= list_len(w)) return;
+        uint32_t facet, seg, cb, bl, fidx = 0;
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
             // the host lays out an item's GPW entries with one facet: read it once per wave
             fidx = __builtin_amdgcn_readfirstlane(w.stream_list[it * (uint64_t)GPW] >> 24);
-            stream = w.stream_list[li] & 0xffffffu;
+            const uint32_t stream = w.stream_list[li] & 0xffffffu;
             if (stream == 0xffffffu) return;   // padding
             wfp = w.facet_args + fidx;
+            uint64_t t = stream;
+            facet = (uint32_t)(t % wfp->g.facets);
+            t /= wfp->g.facets;
+            seg = (uint32_t)(t % w.nseg);
+            t /= w.nseg;
+            cb = (uint32_t)(t % wfp->g.n_chan_blocks);
+            bl = (uint32_t)(t / wfp->g.n_chan_blocks);
         } else {
-            stream = w.stream_list[li];
+            single_stream(w, w.g, li, facet, bl, cb, seg);
         }
         const WArgs &wf = *wfp;
         const GridArgs &a = wf.g;
-        uint64_t t = stream;
-        const uint32_t facet = (uint32_t)(t % a.facets);
-        t /= a.facets;
-        const uint32_t seg = (uint32_t)(t % w.nseg);
-        t /= w.nseg;
-        const uint32_t cb = (uint32_t)(t % a.n_chan_blocks);
-        const uint32_t bl = (uint32_t)(t / a.n_chan_blocks);
         const uint64_t cls_base = ((uint64_t)bl * a.n_chan_blocks + cb) * w.nseg_base;
         const uint32_t b0 = a.bl_start[bl];
         const uint32_t nb = a.bl_start[bl + 1] - b0;
@@ -760,8 +789,11 @@ static uint64_t persistent_blocks(K kernel, uint32_t block, size_t smem, uint64_
 template <int MODE, int PATH, int M, bool BATCH>
 static hipError_t launch_col(const WArgs &w, hipStream_t st) {
     typedef WColGridder<MODE, PATH, M, BATCH> Gd;
-    const uint64_t waves = (w.n_list + Gd::GPW - 1) / Gd::GPW;
+    // device-built lists: the length is on the device, the launch persistent at full occupancy
+    const uint64_t n_list = w.seg_bl ? (uint64_t)1 << 40 : w.n_list;
+    const uint64_t waves = (n_list + Gd::GPW - 1) / Gd::GPW;
     uint64_t blocks = (waves + WCOL_BLOCK / 64 - 1) / (WCOL_BLOCK / 64);
+    if (w.seg_bl && !w.queue) return hipErrorInvalidValue;
     if (!blocks) return hipSuccess;
     const size_t smem = (size_t)(WCOL_BLOCK / 64) * Gd::WAVE_BYTES;
     if (w.queue)
@@ -794,9 +826,12 @@ __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
 template <int MODE, int PATH, int PAIRS, bool BATCH>
 static hipError_t launch_t(const WArgs &w, uint32_t block, hipStream_t st) {
     typedef typename WGridder<MODE, PATH, PAIRS, BATCH>::R_t R_t;
-    uint64_t blocks = (w.n_list + w.streams_per_block - 1) / w.streams_per_block;
+    const uint64_t n_list = w.seg_bl ? (uint64_t)1 << 40 : w.n_list;
+    uint64_t blocks = (n_list + w.streams_per_block - 1) / w.streams_per_block;
+    if (w.seg_bl && !w.queue) return hipErrorInvalidValue;
     if (!blocks) return hipSuccess;
-    const size_t smem = (size_t)w.streams_per_block * REC * sizeof(R_t);
+    const size_t smem =
+        (size_t)w.streams_per_block * std::min<uint32_t>(REC, w.lanes_per_stream) * sizeof(R_t);
     if (w.queue)
         blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, BATCH>, block, smem, blocks);
     wproj_kernel<MODE, PATH, PAIRS, BATCH><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
@@ -844,7 +879,7 @@ void wproj_shape(int M, int min_pairs, uint32_t &block, uint32_t &lanes, uint32_
     pairs = m2 <= 1024 ? 1 : m2 <= 2048 ? 2 : 4;
     pairs = std::max(pairs, min_pairs == 2 ? 2 : min_pairs >= 4 ? 4 : 1);
     const int per = (m2 + pairs - 1) / pairs;
-    // lanes per stream: whole wavefronts, at least one (REC preparers)
+    // lanes per stream: whole wavefronts, at least one (the first REC lanes prepare samples)
     lanes = (uint32_t)std::max(64, (per + 63) / 64 * 64);
     spb = block / lanes;
 }
@@ -953,6 +988,177 @@ hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *pla
     w.facet_args = nullptr;
     w.queue = nullptr;
     w.n_items = 0;
+    w.list_start = w.list_end = nullptr;
+    w.seg_bl = w.seg_first = nullptr;
+    return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
+}
+
+// ---------- device-built class lists of a single-facet-set call ------------------------------
+
+hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
+
+namespace wp {
+
+// Segments of each baseline (SEG rows each, the last one partial).
+__global__ void k_seg_count(const uint32_t *bl_start, uint32_t B, uint32_t *cnt) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    cnt[b] = b < B ? (bl_start[b + 1] - bl_start[b] + SEG - 1) / SEG : 0u;
+}
+
+__global__ void k_seg_fill(const uint32_t *seg_first, uint32_t B, uint32_t *seg_bl) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    for (uint32_t s = seg_first[b]; s < seg_first[b + 1]; ++s) seg_bl[s] = b;
+}
+
+// Class of every (segment, channel block) entry: the widest plane (2 S_p + 1) any of its samples
+// can use (k_stream_class for the segment table); entries beyond the segments get class 0.
+__global__ void k_entry_class(GridArgs a, const uint32_t *seg_first, const uint32_t *seg_bl,
+                              uint64_t entries, const uint32_t *plane_half, uint32_t wplanes,
+                              float wmax_f, uint8_t *cls, uint32_t *val) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= entries) return;
+    val[e] = (uint32_t)e;
+    const uint32_t s = (uint32_t)(e / a.n_chan_blocks);
+    const uint32_t cb = (uint32_t)(e % a.n_chan_blocks);
+    int best = 0;
+    if (s < seg_first[a.baselines]) {
+        const uint32_t bl = seg_bl[s];
+        const uint32_t b0 = a.bl_start[bl], nb = a.bl_start[bl + 1] - b0;
+        const uint32_t r_lo = (s - seg_first[bl]) * SEG, r_hi = min(nb, r_lo + SEG);
+        const uint32_t c_lo = cb * a.chan_per_stream;
+        const uint32_t c_hi = min(a.chan, c_lo + a.chan_per_stream);
+        for (uint32_t r = r_lo; r < r_hi; ++r) {
+            const uint32_t row = a.perm[b0 + r];
+            const uint32_t spw = a.spw[row];
+            if (spw >= a.spw_count) continue;
+            const double w = fabs((double)a.uvw[3 * (size_t)row + 2]);
+            for (uint32_t c = c_lo; c < c_hi; ++c) {
+                const double wl = w / (double)a.wavelengths[spw * a.chan + c];
+                const double pl = rint(wl / (double)wmax_f * (double)(wplanes - 1));
+                if (pl < (double)wplanes) best = max(best, (int)(2 * plane_half[(int)pl] + 1));
+            }
+        }
+    }
+    cls[e] = (uint8_t)best;
+}
+
+// First sorted entry of each class value c (0..256).
+__global__ void k_class_start(const uint8_t *sorted, uint64_t entries, uint32_t *start) {
+    const uint32_t c = threadIdx.x + blockIdx.x * blockDim.x;
+    if (c > 256) return;
+    uint64_t lo = 0, hi = entries;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if ((uint32_t)sorted[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[c] = (uint32_t)lo;
+}
+
+}  // namespace wp
+
+static size_t wl_align(size_t n) { return (n + 255) & ~(size_t)255; }
+
+WListPlan wproj_list_plan(uint32_t rows, uint32_t baselines, uint32_t n_chan_blocks) {
+    WListPlan p{};
+    p.nseg_max = ((uint64_t)rows + wp::SEG - 1) / wp::SEG + baselines;
+    p.entries = p.nseg_max * n_chan_blocks;
+    size_t scan_bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t *)nullptr,
+                                           (uint32_t *)nullptr, (int)baselines + 1);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, p.sort_bytes, (uint8_t *)nullptr,
+                                             (uint8_t *)nullptr, (uint32_t *)nullptr,
+                                             (uint32_t *)nullptr, (int)p.entries, 0, 8);
+    p.sort_bytes = std::max(p.sort_bytes, scan_bytes);
+    size_t o = 0;
+    p.off_seg_first = o;  o += wl_align(((size_t)baselines + 1) * 4);
+    p.off_seg_bl = o;     o += wl_align(p.nseg_max * 4);
+    p.off_cls = o;        o += wl_align(p.entries);
+    p.off_cls_sorted = o; o += wl_align(p.entries);
+    // (val first holds the per-baseline segment counts that seg_first is scanned from)
+    p.off_val = o;        o += wl_align(std::max<uint64_t>(p.entries, (uint64_t)baselines + 1) * 4);
+    p.off_val_sorted = o; o += wl_align(p.entries * 4);
+    p.off_class_start = o; o += wl_align(257 * 4);
+    p.off_queues = o;     o += wl_align(256 * 4);
+    p.off_sort = o;       o += wl_align(p.sort_bytes);
+    p.total = o;
+    return p;
+}
+
+hipError_t wproj_build_lists(const GridArgs &a, const WListPlan &pl, void *ws,
+                             const uint32_t *plane_half, uint32_t wplanes, float wmax_f,
+                             hipStream_t st) {
+    unsigned char *b = static_cast<unsigned char *>(ws);
+    uint32_t *seg_first = reinterpret_cast<uint32_t *>(b + pl.off_seg_first);
+    uint32_t *seg_bl = reinterpret_cast<uint32_t *>(b + pl.off_seg_bl);
+    uint8_t *cls = b + pl.off_cls, *cls_sorted = b + pl.off_cls_sorted;
+    uint32_t *val = reinterpret_cast<uint32_t *>(b + pl.off_val);
+    uint32_t *val_sorted = reinterpret_cast<uint32_t *>(b + pl.off_val_sorted);
+    uint32_t *cstart = reinterpret_cast<uint32_t *>(b + pl.off_class_start);
+    uint32_t *queues = reinterpret_cast<uint32_t *>(b + pl.off_queues);
+    void *tmp = b + pl.off_sort;
+    const uint32_t B = a.baselines;
+    // seg_first (counts, then their exclusive sum in place through val as the scan's input)
+    wp::k_seg_count<<<(B + 1 + 255) / 256, 256, 0, st>>>(a.bl_start, B, val);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = pl.sort_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, val, seg_first, (int)B + 1, st);
+    if (e != hipSuccess) return e;
+    if (B) wp::k_seg_fill<<<(B + 255) / 256, 256, 0, st>>>(seg_first, B, seg_bl);
+    if (pl.entries) {
+        wp::k_entry_class<<<(unsigned)((pl.entries + 255) / 256), 256, 0, st>>>(
+            a, seg_first, seg_bl, pl.entries, plane_half, wplanes, wmax_f, cls, val);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        bytes = pl.sort_bytes;
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, cls, cls_sorted, val, val_sorted,
+                                               (int)pl.entries, 0, 8, st);
+        if (e != hipSuccess) return e;
+    }
+    wp::k_class_start<<<2, 256, 0, st>>>(cls_sorted, pl.entries, cstart);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return fill_u32(queues, 256, 0u, st);
+}
+
+hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void *ws,
+                                    const void *taps, const uint64_t *plane_off,
+                                    const uint32_t *plane_half, const double *plane_norm,
+                                    uint32_t wplanes, uint32_t nf, uint32_t fbase, uint32_t M,
+                                    float wmax_f, int mode, int path, hipStream_t st) {
+    unsigned char *b = static_cast<unsigned char *>(ws);
+    wp::WArgs w;
+    w.g = a;
+    w.taps = taps;
+    w.plane_off = plane_off;
+    w.plane_half = plane_half;
+    w.plane_norm = plane_norm;
+    w.stream_list = reinterpret_cast<const uint32_t *>(b + pl.off_val_sorted);
+    w.n_list = 0;
+    w.wplanes = wplanes;
+    w.nf = nf;
+    w.fbase = fbase;
+    w.M = M;
+    w.wmax_f = wmax_f;
+    w.seg_rows = wp::SEG;
+    w.nseg = 0;
+    w.base_rows = wp::SEG;
+    w.nseg_base = 0;
+    w.filter = 0;
+    w.seg_cls = nullptr;
+    w.facet_args = nullptr;
+    w.queue = reinterpret_cast<uint32_t *>(b + pl.off_queues) + M;
+    w.n_items = 0;
+    const uint32_t *cstart = reinterpret_cast<const uint32_t *>(b + pl.off_class_start);
+    w.list_start = cstart + M;
+    w.list_end = cstart + M + 1;
+    w.seg_bl = reinterpret_cast<const uint32_t *>(b + pl.off_seg_bl);
+    w.seg_first = reinterpret_cast<const uint32_t *>(b + pl.off_seg_first);
     return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
 }
 

@@ -33,7 +33,45 @@ struct WArgs {
     // streams) from this counter until n_items are taken; nullptr: one item per wave/block.
     uint32_t *queue;
     uint32_t n_items;
+    // Single-facet-set launches whose class lists were built on the device (nullptr: the host
+    // built stream_list): the class's entries are stream_list[*list_start .. *list_end), entry
+    // e = segment * n_chan_blocks + channel block, and list index li is entry li / facets with
+    // facet li % facets; segment s belongs to baseline seg_bl[s], whose first segment is
+    // seg_first[baseline].  n_list and n_items then follow from the device counts.
+    const uint32_t *list_start, *list_end;
+    const uint32_t *seg_bl, *seg_first;
 };
+
+// Stream of list index li of a launch without batched facets: facet, baseline, channel block
+// and segment.
+__device__ __forceinline__ void single_stream(const WArgs &w, const GridArgs &a, uint64_t li,
+                                              uint32_t &facet, uint32_t &bl, uint32_t &cb,
+                                              uint32_t &seg) {
+    if (w.seg_bl) {
+        const uint32_t e = w.stream_list[*w.list_start + (uint32_t)(li / a.facets)];
+        facet = (uint32_t)(li % a.facets);
+        cb = e % a.n_chan_blocks;
+        const uint32_t s = e / a.n_chan_blocks;
+        bl = w.seg_bl[s];
+        seg = s - w.seg_first[bl];
+        return;
+    }
+    uint64_t t = w.stream_list[li];
+    facet = (uint32_t)(t % a.facets);
+    t /= a.facets;
+    seg = (uint32_t)(t % w.nseg);
+    t /= w.nseg;
+    cb = (uint32_t)(t % a.n_chan_blocks);
+    bl = (uint32_t)(t / a.n_chan_blocks);
+}
+
+// Streams in the launch's list, and work items of `per` streams.
+__device__ __forceinline__ uint64_t list_len(const WArgs &w) {
+    return w.seg_bl ? (uint64_t)(*w.list_end - *w.list_start) * w.g.facets : w.n_list;
+}
+__device__ __forceinline__ uint64_t list_items(const WArgs &w, uint32_t per) {
+    return w.seg_bl ? (list_len(w) + per - 1) / per : w.n_items;
+}
 
 }  // namespace wp
 
@@ -53,5 +91,26 @@ bool wproj_shape_valid(int M, int shape);
 // Streams per work item of a class launch: GPW for the column kernel (one wave), the block's
 // streams for the pair kernel.
 uint32_t wproj_streams_per_item(int M, int mode, int path, int shape);
+
+// Device-built class lists of a single-facet-set w-projection call (no host round trip): the
+// stream segments of SEG rows per baseline, each (segment, channel block) entry classified by
+// the widest plane its samples use and sorted stably by class; class M's entries start at
+// class_start[M] and end at class_start[M + 1].  Workspace layout and sizes from
+// wproj_list_plan; every array lives in the workspace.
+struct WListPlan {
+    uint64_t nseg_max, entries;          // segment bound, entries = nseg_max * n_chan_blocks
+    size_t off_seg_first, off_seg_bl, off_cls, off_cls_sorted, off_val, off_val_sorted,
+        off_class_start, off_queues, off_sort, sort_bytes, total;
+};
+WListPlan wproj_list_plan(uint32_t rows, uint32_t baselines, uint32_t n_chan_blocks);
+hipError_t wproj_build_lists(const GridArgs &a, const WListPlan &pl, void *ws,
+                             const uint32_t *plane_half, uint32_t wplanes, float wmax_f,
+                             hipStream_t st);
+// One class launch (persistent) over the device-built list of class M.
+hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void *ws,
+                                    const void *taps, const uint64_t *plane_off,
+                                    const uint32_t *plane_half, const double *plane_norm,
+                                    uint32_t wplanes, uint32_t nf, uint32_t fbase, uint32_t M,
+                                    float wmax_f, int mode, int path, hipStream_t st);
 
 }  // namespace bam

@@ -64,8 +64,6 @@ struct State {
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
     std::vector<uint32_t> wplane_half_host;
     uint32_t w_nf = 0, w_fbase = 0;
-    std::vector<uint8_t> wcls_host;
-    std::vector<uint32_t> wlist_host;
     std::vector<normalization_base_type> host_norm;
     std::vector<double> taper_x, taper_y;
     // timing
@@ -555,52 +553,30 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
     }
 
     if (wproj) {
-        // w-projection: streams are SEG-row segments bucketed by their support class
-        const uint32_t SEG = 32;
-        std::vector<uint32_t> bls(B + 1);
-        HIP_TRY(hipMemcpyAsync(bls.data(), blstart, (B + 1) * 4, hipMemcpyDeviceToHost,
-                               g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
-        uint32_t max_nb = 0;
-        for (size_t b = 0; b < B; ++b) max_nb = std::max(max_nb, bls[b + 1] - bls[b]);
-        const uint32_t nsegw = std::max<uint32_t>(1, (max_nb + SEG - 1) / SEG);
-        a.n_segments = nsegw;
-        const uint64_t ncls = (uint64_t)B * a.n_chan_blocks * nsegw;
-        if (ncls * g.F > 0xffffffffull) return fail("too many w-projection streams");
-        a.n_streams = ncls * g.F;
+        // w-projection: streams are segments of 32 rows of a baseline; the device classifies
+        // every (segment, channel block) by its support class and sorts them by class, and
+        // one persistent launch per class width takes its list from there (no host round trip)
         const float wmax_f = (float)p->wmax_est;
         if (!(wmax_f > 0)) return fail("wmax_est must be > 0 for w-projection");
-        HIP_TRY(g.wcls.ensure(ncls));
-        HIP_TRY(launch_stream_class(a, g.wplane_half.as<uint32_t>(), (uint32_t)g.wplanes, wmax_f,
-                                    nsegw, SEG, g.wcls.as<uint8_t>(), g.stream));
-        g.wcls_host.resize(ncls);
-        HIP_TRY(hipMemcpyAsync(g.wcls_host.data(), g.wcls.p, ncls, hipMemcpyDeviceToHost,
-                               g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
-        // counting sort of the streams by class (every facet of a segment shares its class)
-        std::vector<uint64_t> count(256, 0), start(257, 0);
-        for (uint64_t i = 0; i < ncls; ++i) count[g.wcls_host[i]] += g.F;
-        for (int c2 = 0; c2 < 256; ++c2) start[c2 + 1] = start[c2] + (c2 ? count[c2] : 0);
-        g.wlist_host.resize(start[256]);
-        std::vector<uint64_t> fill(start.begin(), start.end() - 1);
-        for (uint64_t i = 0; i < ncls; ++i) {
-            const uint8_t c2 = g.wcls_host[i];
-            if (!c2) continue;
-            for (size_t f = 0; f < g.F; ++f) g.wlist_host[fill[c2]++] = (uint32_t)(i * g.F + f);
-        }
-        HIP_TRY(g.wlist.ensure(std::max<size_t>(4, g.wlist_host.size() * 4)));
-        HIP_TRY(hipMemcpyAsync(g.wlist.p, g.wlist_host.data(), g.wlist_host.size() * 4,
-                               hipMemcpyHostToDevice, g.stream));
+        const WListPlan lp = wproj_list_plan((uint32_t)rows, (uint32_t)B, a.n_chan_blocks);
+        if (lp.entries >= 0x7fffffffull || lp.entries * g.F > 0xffffffffull)
+            return fail("too many w-projection streams");
+        a.n_segments = 0;   // streams come from the device lists
+        a.n_streams = lp.entries * g.F;
+        void *lws = nullptr;
+        HIP_TRY(table(g.wlist, lp.total, &lws));
+        HIP_TRY(wproj_build_lists(a, lp, lws, g.wplane_half.as<uint32_t>(), (uint32_t)g.wplanes,
+                                  wmax_f, g.stream));
         const void *taps = g.path == PATH_FP64 ? g.wtaps_d.p : g.wtaps_f.p;
+        uint32_t max_m = 1;
+        for (uint32_t h : g.wplane_half_host) max_m = std::max(max_m, 2 * h + 1);
         HIP_TRY(hipEventRecord(tc.k0, g.stream));
-        for (int M = 1; M < 256; M += 2) {
-            if (!count[M]) continue;
-            HIP_TRY(launch_wproj(a, taps, g.wplane_off.as<uint64_t>(), g.wplane_half.as<uint32_t>(),
-                                 g.wplane_norm.as<double>(), g.wlist.as<uint32_t>() + start[M],
-                                 (uint32_t)count[M], (uint32_t)g.wplanes, g.w_nf, g.w_fbase,
-                                 (uint32_t)M, wmax_f, mode, g.path, g.stream));
-        }
-        HIP_TRY(hipStreamSynchronize(g.stream));  // wlist_host stays valid until here
+        for (uint32_t M = 1; M <= max_m; M += 2)
+            HIP_TRY(launch_wproj_device_list(a, lp, lws, taps, g.wplane_off.as<uint64_t>(),
+                                             g.wplane_half.as<uint32_t>(),
+                                             g.wplane_norm.as<double>(), (uint32_t)g.wplanes,
+                                             g.w_nf, g.w_fbase, M, wmax_f, mode, g.path,
+                                             g.stream));
     } else {
         const bool specialised = romein_w_groups_per_wave(W, mode, g.path) > 0 &&
                                  !g.opts.reserved[0];
