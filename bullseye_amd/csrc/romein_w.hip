@@ -77,6 +77,19 @@ template <int MODE, int PATH, int W> struct GridderW {
     static constexpr int GSTRIDE = GRAW + ((8 - (GRAW / 4) % 32 + 32) % 32) * 4;
     static constexpr int WAVE_BYTES = GPW * GSTRIDE;
 
+    // groups start on a multiple of 4 lanes, so the QUAD samples of a quad are a lane quad
+    static constexpr bool QUAD_LANES = (G % QUAD) == 0;
+
+    // v of lane q of this lane's quad (DPP quad permutation, no LDS)
+    static __device__ __forceinline__ int quad_lane(int v, int q) {
+        switch (q) {
+            case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xf, 0xf, false);
+            case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xf, 0xf, false);
+            case 2: return __builtin_amdgcn_mov_dpp(v, 0xaa, 0xf, 0xf, false);
+        }
+        return __builtin_amdgcn_mov_dpp(v, 0xff, 0xf, 0xf, false);
+    }
+
     static __device__ __forceinline__ int mod_w(int v) {  // v in (-W, 2W)
         v = v < 0 ? v + W : v;
         return v >= W ? v - W : v;
@@ -457,6 +470,11 @@ template <int MODE, int PATH, int W> struct GridderW {
         int px0 = -1, py0 = -1, pcgi = -1;   // footprint of the accumulated samples
         int ncgi = -1;
 
+        // this lane's sample s = step0 + gl as (outer, inner) indexes of the stream's
+        // serpentine order (stream_sample), advanced by G samples a step without a division
+        const uint32_t inner = max(1u, tin ? nrows : ncb);
+        uint32_t so = (uint32_t)gl / inner, si = (uint32_t)gl - so * inner;
+        const uint32_t dso = (uint32_t)G / inner, dsi = (uint32_t)G - dso * inner;
         for (uint32_t step0 = 0; step0 < nsamp; step0 += G) {
             const uint32_t s = step0 + gl;
             // issue priority: low while preparing (mostly waiting on loads), high while walking
@@ -466,8 +484,8 @@ template <int MODE, int PATH, int W> struct GridderW {
                 R_t rec;
                 rec.cgi = -1;
                 if (s < nsamp) {
-                    uint32_t tr, j;
-                    stream_sample(s, nrows, ncb, tin, tr, j);
+                    const uint32_t i = (so & 1) ? inner - 1 - si : si;
+                    const uint32_t tr = tin ? i : so, j = tin ? so : i;
                     const uint32_t c = c_lo + j;
                     double nw[NC];
                     if (prep(a, gp, stap, sts, schan, b0 + r_lo + tr, jfacet, c, rec, nw)) {
@@ -498,6 +516,30 @@ template <int MODE, int PATH, int W> struct GridderW {
                         rec.tu = rec.tv = (int)zero_run;
                     }
                 }
+                if constexpr (KAHAN && QUAD_LANES) {
+                    // the quad's lanes are a hardware lane quad: each reads the quad's records
+                    // from its neighbours' registers, and the first writes the quad's decision
+                    // (below) into its own record
+                    int fc = -1, fx = 0, fy = 0;
+                    bool same = true;
+#pragma unroll
+                    for (int q = 0; q < QUAD; ++q) {
+                        const int cq = quad_lane(rec.cgi, q);
+                        const int xq = quad_lane(rec.x0, q), yq = quad_lane(rec.y0, q);
+                        if (cq < 0) continue;
+                        if (fc < 0) {
+                            fc = cq;
+                            fx = xq;
+                            fy = yq;
+                        } else {
+                            same = same && cq == fc && xq == fx && yq == fy;
+                        }
+                    }
+                    rec.qf = same ? 1 : 0;
+                    rec.qx0 = fx;
+                    rec.qy0 = fy;
+                    rec.qcgi = fc;
+                }
                 grp[gl] = rec;
             }
             __builtin_amdgcn_s_setprio(3);
@@ -506,9 +548,9 @@ template <int MODE, int PATH, int W> struct GridderW {
             int kk = 0;
             if constexpr (KAHAN) {
                 const int nq = kmax / QUAD;
-                // the leader of each full quad decides, once, whether the quad's valid samples
-                // share one footprint
-                if ((gl % QUAD) == 0 && gl / QUAD < nq) {
+                // otherwise the leader of each full quad decides, once, whether the quad's valid
+                // samples share one footprint
+                if (!QUAD_LANES && (gl % QUAD) == 0 && gl / QUAD < nq) {
                     int cg = -1, x0 = 0, y0 = 0;
                     bool same = true;
 #pragma unroll
@@ -529,7 +571,7 @@ template <int MODE, int PATH, int W> struct GridderW {
                     grp[gl].qy0 = y0;
                     grp[gl].qcgi = cg;
                 }
-                wave_sync();
+                if (!QUAD_LANES) wave_sync();
                 Run y;
                 bool open = false;
                 for (int q = 0; q < nq; ++q) {
@@ -574,6 +616,12 @@ template <int MODE, int PATH, int W> struct GridderW {
                 add_one(grp[kk], stap, ua, c0, acc);
             }
             wave_sync();
+            si += dsi;
+            so += dso;
+            if (si >= inner) {
+                si -= inner;
+                ++so;
+            }
         }
         if (pcgi >= 0) {
             const int xo = px0 + mod_w(ua - px0 % W);
