@@ -40,7 +40,7 @@ constexpr uint32_t LIST_PAD = 0xffffffu;   // stream id of a no-op work-list ent
 
 // Streams per column-kernel item of class width M (1 where the class has no column kernel).
 uint32_t column_run(int M, int kind, int path) {
-    return M <= 15 ? wproj_streams_per_item(M, kind, path, WSHAPE_COLUMN) : 1u;
+    return M <= wp::WCOL_MAX ? wproj_streams_per_item(M, kind, path, WSHAPE_COLUMN) : 1u;
 }
 
 struct KernelSet {
@@ -48,10 +48,13 @@ struct KernelSet {
     uint32_t wplanes = 0, S = 0, OS = 0, csz = 0, nf = 0, fbase = 0;
     float wmax = 0;
     std::vector<uint32_t> half;
-    DevBuf taps_f, taps_d, off, halfd, norm;
+    DevBuf taps_f, taps_d, off, halfd, norm, ctaps_f, ctaps_d, coff;
     void release() {
         taps_f.release();
         taps_d.release();
+        ctaps_f.release();
+        ctaps_d.release();
+        coff.release();
         off.release();
         halfd.release();
         norm.release();
@@ -671,6 +674,8 @@ int build_plan(gridding_parameters *p, int mode, const bullseye_batch_facet *fac
         fa.off_y = (double)(fc.ny / 2 - ks.S);
         w.taps = lv.path == PATH_FP64 ? ks.taps_d.p : ks.taps_f.p;
         w.plane_off = ks.off.as<uint64_t>();
+        w.ctaps = lv.path == PATH_FP64 ? ks.ctaps_d.p : ks.ctaps_f.p;
+        w.cplane_off = ks.coff.as<uint64_t>();
         w.plane_half = ks.halfd.as<uint32_t>();
         w.plane_norm = ks.norm.as<double>();
         w.wplanes = ks.wplanes;
@@ -947,7 +952,7 @@ int bullseye_batch_kernel_set(uint32_t id, const void *conv, uint32_t wplanes,
             ks.half[q] = half_support[q];
         }
     if (build_wtaps(conv, lv.device_resident, wplanes, ks.S, ks.OS, ks.half, ks.taps_f, ks.taps_d,
-                    ks.off, ks.halfd, ks.norm, ks.nf, ks.fbase))
+                    ks.off, ks.halfd, ks.norm, ks.ctaps_f, ks.ctaps_d, ks.coff, ks.nf, ks.fbase))
         return 1;
     ks.defined = true;
     return 0;

@@ -17,11 +17,6 @@ hipError_t invert_jones(const visibility_base_type *in, void *out, uint64_t n, b
 hipError_t launch_stream_class(const GridArgs &a, const uint32_t *plane_half, uint32_t wplanes,
                                float wmax_f, uint32_t nseg_max, uint32_t seg_rows, uint8_t *cls,
                                hipStream_t st);
-hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,
-                        const uint32_t *plane_half, const double *plane_norm,
-                        const uint32_t *stream_list, uint32_t n_list, uint32_t wplanes,
-                        uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
-                        hipStream_t st);
 int romein_max_width();
 // time_inner: the flag followed by the measurement's partial sums (walk_order_bytes() in all)
 size_t walk_order_bytes();
@@ -120,8 +115,8 @@ int check_device_errors();
 // Re-laid w-projection taps of a kernel stack (conv as initLibrary's).
 int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_t OS,
                 const std::vector<uint32_t> &halves, DevBuf &taps_f, DevBuf &taps_d,
-                DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, uint32_t &w_nf,
-                uint32_t &w_fbase);
+                DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, DevBuf &ctaps_f,
+                DevBuf &ctaps_d, DevBuf &cplane_off, uint32_t &w_nf, uint32_t &w_fbase);
 // Free batch plans and kernel sets (releaseLibrary).
 void batch_release_all();
 

@@ -35,7 +35,10 @@ constexpr int REC = 128;       // samples prepared per stream per step (at most 
 template <typename T, int NC> struct WRec {
     int x0, y0, xm, ym;     // class-footprint origin and its residues mod M
     int off, wp, cgi;       // off = S_M - S_p (first active tap), plane width W_p, grid slice
-    uint32_t tb;            // first tap of the sample's W_p x W_p block
+    // pair kernels: first tap of the sample's W_p x W_p block; column kernel: the tap of
+    // window column 0 for v residue 0 in the class's column table
+    uint32_t tb;
+    int ru0;                // column kernel: u residue of the window's first column
     cpx<T> vis[NC];
 };
 
@@ -81,7 +84,9 @@ template <int MODE, int PATH> struct WPrep {
     // Sample (sorted row ri, channel c).  The checks and their order (and so the errors raised)
     // are those of a gather straight from the row arrays: spw range, enabled channel, grid
     // slice range, row flag and field, then the Jones ids.
-    static __device__ __forceinline__ bool prep(const WArgs &w, int M, uint32_t facet,
+    // col: the record addresses the class's column table (column kernel), else the plane's
+    // W_p x W_p blocks.
+    static __device__ __forceinline__ bool prep(const WArgs &w, int M, bool col, uint32_t facet,
                                                 uint32_t ri, uint32_t c, R_t &r, double nw[NC]) {
         const GridArgs &a = w.g;
         const RowRec rr = load_row_record(a.rowrec + ri);
@@ -201,7 +206,17 @@ template <int MODE, int PATH> struct WPrep {
         const int ru_ = fu - (int)w.fbase, rv_ = fv - (int)w.fbase;
         r.cgi = (int)cgi;
         r.wp = 2 * sp + 1;
-        r.tb = (uint32_t)(w.plane_off[plane] + ((uint64_t)rv_ * w.nf + ru_) * (uint64_t)(r.wp * r.wp));
+        const uint64_t q = (uint64_t)rv_ * w.nf + ru_;
+        if (col) {
+            // window row tv of v residue r is (r - rv0) mod M: the 2M table rows of a block
+            // read from row M - rv0 on are the rows of residues 0 .. M-1 in order
+            const int rv0 = (r.ym + r.off) % M;
+            r.ru0 = (r.xm + r.off) % M;
+            r.tb = (uint32_t)(w.cplane_off[(M >> 1) * w.wplanes + plane] +
+                              (q * 2 + 1) * (uint64_t)(M * M) - (uint64_t)(rv0 * M));
+        } else {
+            r.tb = (uint32_t)(w.plane_off[plane] + q * (uint64_t)(r.wp * r.wp));
+        }
 #pragma unroll
         for (int k = 0; k < NC; ++k) r.vis[k] = vis[k];
         const double ts = w.plane_norm[((size_t)plane * w.nf + rv_) * w.nf + ru_];
@@ -259,13 +274,16 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
     }
 
     static __device__ void run_item(const WArgs &w, uint64_t item, unsigned char *smem) {
-        const int M = (int)w.M;
         const int L = (int)w.lanes_per_stream;
         const int slot = threadIdx.x / L;            // stream slot in the block
         const int sl = threadIdx.x - slot * L;       // lane within the stream
         const bool in_slot = slot < (int)w.streams_per_block;
         const uint64_t li = item * w.streams_per_block + slot;
         bool live = in_slot && li < list_len(w);
+        // width of this stream: the launch's, or its own class in a merged list
+        const int M = (!BATCH && w.list_cls && live)
+                          ? (int)w.list_cls[*w.list_start + (uint32_t)(li / w.g.facets)]
+                          : (int)w.M;
         uint32_t stream = 0, fidx = 0;
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
@@ -357,7 +375,8 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                     double nw[NC];
                     const bool mine =
                         !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
-                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, b0 + r_lo + tr, c, rec, nw)) {
+                    if (mine &&
+                        WPrep<MODE, PATH>::prep(wf, M, false, facet, b0 + r_lo + tr, c, rec, nw)) {
                         ++cnt;
                         taps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -442,8 +461,9 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
 // v-residues.  Per sample a lane issues its M tap loads together and does M complex MACs, so
 // the per-sample bookkeeping (record read, ownership move) is shared by M taps and the loads'
 // latency overlaps.  FP32 compensated sums take one Kahan step per run of QUAD samples on one
-// footprint (products chained by FMA onto -c).
-constexpr int WCOL_MAX = 15;
+// footprint (products chained by FMA onto -c).  The taps of a lane's column come from the
+// class's column table at fixed strides from one address, so the loads need no per-tap
+// address, window test or select.
 constexpr int WCOL_BLOCK = 256;
 constexpr int QUAD = 4;
 
@@ -465,7 +485,11 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
     // record arrays of consecutive groups start 8 LDS banks apart
     static constexpr int GRAW = (G * (int)sizeof(R_t) + 15) & ~15;
     static constexpr int GSTRIDE = GRAW + ((8 - (GRAW / 4) % 32 + 32) % 32) * 4;
-    static constexpr int WAVE_BYTES = GPW * GSTRIDE;
+    // LDS staging of a column that leaves the footprint: M rows x NC correlations x (re, im)
+    static constexpr int STAGE = 2 * M * NC;
+    static constexpr int WAVE_BYTES = GPW * GSTRIDE + GPW * STAGE * 8;
+    static_assert(G < 64, "a group's lanes form a 64-bit ballot slice");
+    static constexpr uint64_t GMASK = (1ull << G) - 1;
     typedef acc1<T, KAHAN> Acc[M][CPL][2];
 
     static __device__ __forceinline__ int mod_m(int v) {  // v in (-M, 2M)
@@ -496,20 +520,62 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
     }
 
     // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.
-    static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int ua,
-                                                   int c0, int x0, int y0, int cgi, int &px0,
-                                                   int &py0, int &pcgi, Acc &acc) {
+    // Rows that leave (and every row when the slice changes or the footprint jumps by more
+    // than two columns) are flushed by all lanes of the group, adjacent cells of a grid row
+    // together.  The other rows of a column that leaves go through the group's LDS staging
+    // area, so that the real and imaginary parts (and the correlations) of a cell are added by
+    // adjacent lanes of one instruction instead of one lane issuing a request per value.
+    static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int gw,
+                                                   int gl, int ua, int c0, int x0, int y0,
+                                                   int cgi, int &px0, int &py0, int &pcgi,
+                                                   Acc &acc, double *stage) {
         if (x0 == px0 && y0 == py0 && cgi == pcgi) return;
         if (pcgi >= 0) {
-            const int xo = px0 + mod_m(ua - px0 % M);
+            const int pxm = px0 % M, pym = py0 % M, ym = y0 % M;
+            const int xo = px0 + mod_m(ua - pxm);
             const int xn = x0 + mod_m(ua - x0 % M);
-            const bool all = (cgi != pcgi) || (xn != xo);
-            const int pym = py0 % M, ym = y0 % M;
+            const bool every = cgi != pcgi || x0 - px0 > 2 || px0 - x0 > 2;
 #pragma unroll
             for (int r = 0; r < M; ++r) {
                 const int yo = py0 + mod_m(r - pym);
                 const int yn = y0 + mod_m(r - ym);
-                if (all || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+                if (every || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
+            }
+            bool pend = !every && xn != xo;
+            for (;;) {
+                const uint64_t gm = (__ballot(pend) >> (gw * G)) & GMASK;
+                if (!__ballot(gm != 0)) break;
+                const int lead = gm ? __ffsll((unsigned long long)gm) - 1 : -1;
+                const int lua = lead >= 0 ? lead % M : 0;
+                if (pend && ua == lua) {   // the NSPLIT lanes of the group's first leaving column
+#pragma unroll
+                    for (int r = 0; r < M; ++r)
+#pragma unroll
+                        for (int k = 0; k < CPL; ++k) {
+                            stage[(r * NC + c0 + k) * 2] = acc[r][k][0].value();
+                            stage[(r * NC + c0 + k) * 2 + 1] = acc[r][k][1].value();
+                            acc[r][k][0].zero();
+                            acc[r][k][1].zero();
+                        }
+                    pend = false;
+                }
+                wave_sync();
+                if (lead >= 0) {
+                    const int lxo = px0 + mod_m(lua - pxm);
+                    const size_t col = (((size_t)facet * a.cube + pcgi) * a.ny) * a.nx + lxo;
+#pragma unroll
+                    for (int i = 0; i < 2 * CPL; ++i) {
+                        const int j = gl + i * G;
+                        const double v = stage[j];
+                        if (v != 0.0) {
+                            const int yo = py0 + mod_m(j / (2 * NC) - pym);
+                            unsafeAtomicAdd(a.grid + ((col + (size_t)yo * a.nx) * a.pstride) * 2 +
+                                                (j % (2 * NC)),
+                                            v);
+                        }
+                    }
+                }
+                wave_sync();
             }
         }
         px0 = x0;
@@ -519,18 +585,16 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
 
     // The lane's column of the sample's taps in v-residue order (zero outside the plane's
     // W_p x W_p window); false when the lane's column is outside the window.
+    // The column's taps are M entries M apart in the class's column table, window column tu
+    // from the sample's v rotation on (zeros outside the window rows); the lanes of a group
+    // read adjacent entries.
     static __device__ __forceinline__ bool column(const R_t &rec, const cpx<T> *taps, int ua,
                                                   cpx<T> (&tp)[M]) {
-        const int tu = mod_m(ua - rec.xm) - rec.off;
-        if ((unsigned)tu >= (unsigned)rec.wp) return false;
-        const uint32_t base = rec.tb + (uint32_t)tu;
+        const int tu = ua - rec.ru0 + (ua < rec.ru0 ? M : 0);
+        if (tu >= rec.wp) return false;
+        const cpx<T> *col = taps + (rec.tb + (uint32_t)tu);
 #pragma unroll
-        for (int r = 0; r < M; ++r) {
-            const int tv = mod_m(r - rec.ym) - rec.off;
-            const bool ok = (unsigned)tv < (unsigned)rec.wp;
-            const cpx<T> t = taps[ok ? base + (uint32_t)(tv * rec.wp) : rec.tb];
-            tp[r] = ok ? t : cpx<T>{T(0), T(0)};
-        }
+        for (int r = 0; r < M; ++r) tp[r] = col[r * M];
         return true;
     }
 
@@ -664,7 +728,8 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
         const int ua = gl % M;          // u residue of this lane
         const int c0 = (gl / M) * CPL;  // first correlation of this lane
         R_t *grp = reinterpret_cast<R_t *>(swave + gw * GSTRIDE);
-        const cpx<T> *taps = reinterpret_cast<const cpx<T> *>(wf.taps);
+        double *stage = reinterpret_cast<double *>(swave + GPW * GSTRIDE) + gw * STAGE;
+        const cpx<T> *taps = reinterpret_cast<const cpx<T> *>(wf.ctaps);
 
         Acc acc;
 #pragma unroll
@@ -689,7 +754,8 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
                     double nw[NC];
                     const bool mine =
                         !w.filter || wf.seg_cls[cls_base + (r_lo + tr) / w.base_rows] == (uint8_t)M;
-                    if (mine && WPrep<MODE, PATH>::prep(wf, M, facet, b0 + r_lo + tr, c, rec, nw)) {
+                    if (mine &&
+                        WPrep<MODE, PATH>::prep(wf, M, true, facet, b0 + r_lo + tr, c, rec, nw)) {
                         ++cnt;
                         ntaps += (unsigned long long)(rec.wp * rec.wp);
                         if constexpr (MODE != MODE_PSF) {
@@ -710,6 +776,7 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
                     rec.off = 0;
                     rec.wp = 0;
                     rec.tb = 0;
+                    rec.ru0 = 0;
 #pragma unroll
                     for (int k = 0; k < NC; ++k) rec.vis[k] = {T(0), T(0)};
                 }
@@ -737,7 +804,8 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
                     }
                     if (!same) break;
                     if (cg >= 0) {
-                        move_to(a, facet, ua, c0, x0, y0, cg, px0, py0, pcgi, acc);
+                        move_to(a, facet, gw, gl, ua, c0, x0, y0, cg, px0, py0, pcgi, acc,
+                                stage);
                         add_quad(grp + kk, taps, ua, c0, acc);
                     }
                     kk += QUAD;
@@ -746,7 +814,8 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
             for (; kk < kmax; ++kk) {
                 const int cg = grp[kk].cgi;
                 if (cg < 0) continue;
-                move_to(a, facet, ua, c0, grp[kk].x0, grp[kk].y0, cg, px0, py0, pcgi, acc);
+                move_to(a, facet, gw, gl, ua, c0, grp[kk].x0, grp[kk].y0, cg, px0, py0, pcgi, acc,
+                        stage);
                 add_one(grp[kk], taps, ua, c0, acc);
             }
             wave_sync();
@@ -762,7 +831,12 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
 };
 
 template <int MODE, int PATH, int M, bool BATCH>
-__global__ __launch_bounds__(WCOL_BLOCK) void wcol_kernel(WArgs w) {
+#ifdef WCOL_WPE
+#define WCOL_ATTR __attribute__((amdgpu_waves_per_eu(WCOL_WPE, 8)))
+#else
+#define WCOL_ATTR
+#endif
+__global__ __launch_bounds__(WCOL_BLOCK) WCOL_ATTR void wcol_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
     typedef WColGridder<MODE, PATH, M, BATCH> Gd;
     Gd::run(w, smem + (threadIdx.x >> 6) * Gd::WAVE_BYTES);
@@ -961,38 +1035,6 @@ size_t wproj_record_bytes(int mode, int path) {
     return 32 + 2 * t * nc;
 }
 
-hipError_t launch_wproj(const GridArgs &a, const void *taps, const uint64_t *plane_off,
-                        const uint32_t *plane_half, const double *plane_norm,
-                        const uint32_t *stream_list, uint32_t n_list, uint32_t wplanes,
-                        uint32_t nf, uint32_t fbase, uint32_t M, float wmax_f, int mode, int path,
-                        hipStream_t st) {
-    wp::WArgs w;
-    w.g = a;
-    w.taps = taps;
-    w.plane_off = plane_off;
-    w.plane_half = plane_half;
-    w.plane_norm = plane_norm;
-    w.stream_list = stream_list;
-    w.n_list = n_list;
-    w.wplanes = wplanes;
-    w.nf = nf;
-    w.fbase = fbase;
-    w.M = M;
-    w.wmax_f = wmax_f;
-    w.seg_rows = wp::SEG;
-    w.nseg = a.n_segments;
-    w.base_rows = wp::SEG;
-    w.nseg_base = a.n_segments;
-    w.filter = 0;
-    w.seg_cls = nullptr;
-    w.facet_args = nullptr;
-    w.queue = nullptr;
-    w.n_items = 0;
-    w.list_start = w.list_end = nullptr;
-    w.seg_bl = w.seg_first = nullptr;
-    return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
-}
-
 // ---------- device-built class lists of a single-facet-set call ------------------------------
 
 hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
@@ -1128,14 +1170,19 @@ hipError_t wproj_build_lists(const GridArgs &a, const WListPlan &pl, void *ws,
 
 hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void *ws,
                                     const void *taps, const uint64_t *plane_off,
+                                    const void *ctaps, const uint64_t *cplane_off,
                                     const uint32_t *plane_half, const double *plane_norm,
                                     uint32_t wplanes, uint32_t nf, uint32_t fbase, uint32_t M,
-                                    float wmax_f, int mode, int path, hipStream_t st) {
+                                    uint32_t M_hi, float wmax_f, int mode, int path,
+                                    hipStream_t st) {
     unsigned char *b = static_cast<unsigned char *>(ws);
+    if (M_hi < M || (M_hi > M && M_hi <= (uint32_t)wp::WCOL_MAX)) return hipErrorInvalidValue;
     wp::WArgs w;
     w.g = a;
     w.taps = taps;
     w.plane_off = plane_off;
+    w.ctaps = ctaps;
+    w.cplane_off = cplane_off;
     w.plane_half = plane_half;
     w.plane_norm = plane_norm;
     w.stream_list = reinterpret_cast<const uint32_t *>(b + pl.off_val_sorted);
@@ -1143,7 +1190,7 @@ hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void
     w.wplanes = wplanes;
     w.nf = nf;
     w.fbase = fbase;
-    w.M = M;
+    w.M = M_hi;
     w.wmax_f = wmax_f;
     w.seg_rows = wp::SEG;
     w.nseg = 0;
@@ -1156,9 +1203,10 @@ hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void
     w.n_items = 0;
     const uint32_t *cstart = reinterpret_cast<const uint32_t *>(b + pl.off_class_start);
     w.list_start = cstart + M;
-    w.list_end = cstart + M + 1;
+    w.list_end = cstart + M_hi + 1;
     w.seg_bl = reinterpret_cast<const uint32_t *>(b + pl.off_seg_bl);
     w.seg_first = reinterpret_cast<const uint32_t *>(b + pl.off_seg_first);
+    w.list_cls = M_hi > M ? b + pl.off_cls_sorted : nullptr;
     return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
 }
 

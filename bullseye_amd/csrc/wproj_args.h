@@ -6,12 +6,21 @@
 namespace bam {
 namespace wp {
 
+constexpr int WCOL_MAX = 15;                           // widest class of the column kernel
+constexpr int WCOL_CLASSES = (WCOL_MAX + 1) / 2;       // its classes 1, 3, .., WCOL_MAX
+
 struct WArgs {
     GridArgs g;                        // shared fields (vis, uvw, flags, grid, norm, ...)
     const void *taps;                  // re-laid taps, complex acc_t
     const uint64_t *plane_off;         // [wplanes] element offset of plane p's blocks
     const uint32_t *plane_half;        // [wplanes] S_p
     const double *plane_norm;          // [wplanes][nf][nf] sum of Re(taps) per block
+    // Column tables of the classes M = 1, 3, .., WCOL_MAX (column kernel): cplane_off[(M >> 1)
+    // * wplanes + p] is the first tap of plane p's table in class M (planes with W_p <= M).
+    // Per (frac_v, frac_u) block 2M rows of M taps: entry (j, t) holds window row j mod M of
+    // column t, zero where j mod M >= W_p or t >= W_p.
+    const void *ctaps;                 // complex acc_t
+    const uint64_t *cplane_off;
     const uint32_t *stream_list;       // streams of this class
     uint32_t n_list;
     uint32_t wplanes, nf, fbase;       // fraction index range [fbase, fbase + nf)
@@ -40,6 +49,10 @@ struct WArgs {
     // seg_first[baseline].  n_list and n_items then follow from the device counts.
     const uint32_t *list_start, *list_end;
     const uint32_t *seg_bl, *seg_first;
+    // Device lists holding several classes (pair kernel, lanes shaped for the widest, M): the
+    // class of sorted list position i is list_cls[i], and each workgroup's stream takes its own
+    // width; nullptr: every stream has width M.
+    const uint8_t *list_cls;
 };
 
 // Stream of list index li of a launch without batched facets: facet, baseline, channel block
@@ -109,8 +122,10 @@ hipError_t wproj_build_lists(const GridArgs &a, const WListPlan &pl, void *ws,
 // One class launch (persistent) over the device-built list of class M.
 hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void *ws,
                                     const void *taps, const uint64_t *plane_off,
+                                    const void *ctaps, const uint64_t *cplane_off,
                                     const uint32_t *plane_half, const double *plane_norm,
                                     uint32_t wplanes, uint32_t nf, uint32_t fbase, uint32_t M,
-                                    float wmax_f, int mode, int path, hipStream_t st);
+                                    uint32_t M_hi, float wmax_f, int mode, int path,
+                                    hipStream_t st);
 
 }  // namespace bam

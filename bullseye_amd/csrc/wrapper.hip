@@ -44,6 +44,11 @@ struct State {
     bullseye_options opts{};
     std::string err;
     hipStream_t stream = nullptr;
+    // w-projection class launches run on the stream and WSIDE side streams (default priority),
+    // forked after the class lists are built and joined before the call's end
+    static constexpr int WSIDE = 3;
+    hipStream_t wside[WSIDE] = {};
+    hipEvent_t wfork = nullptr, wjoin[WSIDE] = {};
     // geometry fixed at initLibrary
     size_t F = 0, cube = 0, P = 0, pstride = 0, nx = 0, ny = 0, S = 0, OS = 0, csz = 0;
     size_t sfC = 0, wplanes = 1;
@@ -61,6 +66,7 @@ struct State {
         dft_comp;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
     DevBuf wtaps_f, wtaps_d, wplane_off, wplane_half, wplane_norm, wcls, wlist;
+    DevBuf wctaps_f, wctaps_d, wcplane_off;   // column tables of the narrow classes
     std::vector<uint32_t> wplane_half_req;   // from bullseye_set_wplane_support
     std::vector<uint32_t> wplane_half_host;
     uint32_t w_nf = 0, w_fbase = 0;
@@ -568,15 +574,41 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
         HIP_TRY(wproj_build_lists(a, lp, lws, g.wplane_half.as<uint32_t>(), (uint32_t)g.wplanes,
                                   wmax_f, g.stream));
         const void *taps = g.path == PATH_FP64 ? g.wtaps_d.p : g.wtaps_f.p;
+        const void *ctaps = g.path == PATH_FP64 ? g.wctaps_d.p : g.wctaps_f.p;
         uint32_t max_m = 1;
         for (uint32_t h : g.wplane_half_host) max_m = std::max(max_m, 2 * h + 1);
         HIP_TRY(hipEventRecord(tc.k0, g.stream));
-        for (uint32_t M = 1; M <= max_m; M += 2)
-            HIP_TRY(launch_wproj_device_list(a, lp, lws, taps, g.wplane_off.as<uint64_t>(),
-                                             g.wplane_half.as<uint32_t>(),
-                                             g.wplane_norm.as<double>(), (uint32_t)g.wplanes,
-                                             g.w_nf, g.w_fbase, M, wmax_f, mode, g.path,
-                                             g.stream));
+        // one persistent launch per class width, dealt over the stream and the side streams so
+        // that a launch's tail and the small classes overlap other classes' work
+        HIP_TRY(hipEventRecord(g.wfork, g.stream));
+        for (int i = 0; i < State::WSIDE; ++i) HIP_TRY(hipStreamWaitEvent(g.wside[i], g.wfork, 0));
+        // Classes wider than 31 hold few samples each: one launch per lane shape (two or four
+        // residue pairs per lane) walks all of them, each stream at its own width, instead of
+        // one latency-bound launch per class.  They go first, so that their long streams overlap
+        // the wide-parallel launches of the narrower classes, which follow widest first.
+        std::vector<std::pair<uint32_t, uint32_t>> order;
+        if (max_m > 45) order.push_back({47, max_m});
+        if (max_m > 31) order.push_back({33, std::min<uint32_t>(max_m, 45)});
+        for (int M = (int)std::min<uint32_t>(max_m, 31); M >= 1; M -= 2)
+            order.push_back({(uint32_t)M, (uint32_t)M});
+        hipError_t le = hipSuccess;
+        for (size_t i = 0; i < order.size() && le == hipSuccess; ++i) {
+            const int turn = (int)(i % (State::WSIDE + 1));
+            le = launch_wproj_device_list(a, lp, lws, taps, g.wplane_off.as<uint64_t>(), ctaps,
+                                          g.wcplane_off.as<uint64_t>(),
+                                          g.wplane_half.as<uint32_t>(),
+                                          g.wplane_norm.as<double>(), (uint32_t)g.wplanes,
+                                          g.w_nf, g.w_fbase, order[i].first, order[i].second,
+                                          wmax_f, mode, g.path,
+                                          turn ? g.wside[turn - 1] : g.stream);
+        }
+        // joined also after a failed launch: later work on the stream (the next call's class
+        // lists) must not overtake what was launched
+        for (int i = 0; i < State::WSIDE; ++i) {
+            HIP_TRY(hipEventRecord(g.wjoin[i], g.wside[i]));
+            HIP_TRY(hipStreamWaitEvent(g.stream, g.wjoin[i], 0));
+        }
+        HIP_TRY(le);
     } else {
         const bool specialised = romein_w_groups_per_wave(W, mode, g.path) > 0 &&
                                  !g.opts.reserved[0];
@@ -614,8 +646,8 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
 // rounding gives fractions in [OS/2, 3 OS/2]), plus the Re-sum of every block.
 int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_t OS,
                 const std::vector<uint32_t> &halves, DevBuf &taps_f, DevBuf &taps_d,
-                DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, uint32_t &w_nf,
-                uint32_t &w_fbase) {
+                DevBuf &plane_off, DevBuf &plane_half, DevBuf &plane_norm, DevBuf &ctaps_f,
+                DevBuf &ctaps_d, DevBuf &cplane_off, uint32_t &w_nf, uint32_t &w_fbase) {
     const size_t csz = (2 * S + 3) + (2 * S + 2) * (OS - 1);
     w_fbase = (uint32_t)(OS / 2);
     w_nf = (uint32_t)(OS + 2);
@@ -661,6 +693,47 @@ int build_wtaps(const void *conv, bool conv_on_device, size_t P, size_t S, size_
                 nrm[(q * nf + fv) * nf + fu] = sum;
             }
     }
+    // column tables of the classes M <= WCOL_MAX (wp::WArgs::ctaps): per plane no wider than
+    // M and block, 2M rows of M taps, entry (j, t) window row j mod M of column t (zero
+    // beyond W_p)
+    constexpr int NCL = wp::WCOL_CLASSES;
+    std::vector<uint64_t> coff((size_t)NCL * P, ~0ull);
+    uint64_t cn = 0;
+    for (size_t ci = 0; ci < (size_t)NCL; ++ci)
+        for (size_t q = 0; q < P; ++q) {
+            const size_t wq = 2 * halves[q] + 1, M = 2 * ci + 1;
+            if (wq > M) continue;
+            coff[ci * P + q] = cn;
+            cn += nf * nf * 2 * M * M;
+        }
+    if (cn > 0xffffffffull) return fail("w-projection column tables exceed 2^32 taps");
+    std::vector<double> cd(2 * cn, 0.0);
+    std::vector<float> cf(2 * cn, 0.0f);
+    for (size_t ci = 0; ci < (size_t)NCL; ++ci)
+        for (size_t q = 0; q < P; ++q) {
+            const size_t wq = 2 * halves[q] + 1, M = 2 * ci + 1;
+            if (wq > M) continue;
+            for (size_t blk = 0; blk < nf * nf; ++blk)
+                for (size_t t = 0; t < wq; ++t)
+                    for (size_t j = 0; j < 2 * M; ++j) {
+                        const size_t row = j % M;
+                        if (row >= wq) continue;
+                        const size_t src = off[q] + (blk * wq + row) * wq + t;
+                        const size_t dst = coff[ci * P + q] + (blk * 2 * M + j) * M + t;
+                        cd[2 * dst] = td[2 * src];
+                        cd[2 * dst + 1] = td[2 * src + 1];
+                        cf[2 * dst] = tf[2 * src];
+                        cf[2 * dst + 1] = tf[2 * src + 1];
+                    }
+        }
+    HIP_TRY(ctaps_d.ensure(std::max<size_t>(cd.size(), 2) * 8));
+    HIP_TRY(ctaps_f.ensure(std::max<size_t>(cf.size(), 2) * 4));
+    HIP_TRY(cplane_off.ensure(coff.size() * 8));
+    if (cn) {
+        HIP_TRY(hipMemcpy(ctaps_d.p, cd.data(), cd.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctaps_f.p, cf.data(), cf.size() * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(cplane_off.p, coff.data(), coff.size() * 8, hipMemcpyHostToDevice));
     HIP_TRY(taps_d.ensure(td.size() * 8));
     HIP_TRY(taps_f.ensure(tf.size() * 4));
     HIP_TRY(plane_off.ensure(P * 8));
@@ -773,9 +846,18 @@ static void release_resources() {
                       &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.in_fc, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
+                      &g.wctaps_f, &g.wctaps_d, &g.wcplane_off,
                       &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
     for (DevBuf *b : bufs) b->release();
     batch_release_all();
+    for (int i = 0; i < State::WSIDE; ++i) {
+        if (g.wside[i]) (void)hipStreamDestroy(g.wside[i]);
+        if (g.wjoin[i]) (void)hipEventDestroy(g.wjoin[i]);
+        g.wside[i] = nullptr;
+        g.wjoin[i] = nullptr;
+    }
+    if (g.wfork) (void)hipEventDestroy(g.wfork);
+    g.wfork = nullptr;
     if (g.stream) (void)hipStreamDestroy(g.stream);
     g.stream = nullptr;
 }
@@ -803,6 +885,11 @@ static int init_impl(gridding_parameters *p) {
         g_scratch_ptr[g.device] = static_cast<unsigned char *>(ptr);
     }
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&g.wfork, hipEventDisableTiming));
+    for (int i = 0; i < State::WSIDE; ++i) {
+        HIP_TRY(hipStreamCreateWithFlags(&g.wside[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&g.wjoin[i], hipEventDisableTiming));
+    }
     g.F = std::max<size_t>(1, p->num_facet_centres);
     g.cube = std::max<size_t>(1, p->cube_channel_dim_size);
     g.P = std::max<size_t>(1, p->number_of_polarization_terms_being_gridded);
@@ -910,7 +997,8 @@ static int init_impl(gridding_parameters *p) {
                 return fail("w-projection plane %zu: half support %u exceeds 31", q,
                             g.wplane_half_host[q]);
         if (build_wtaps(p->conv, dev_mode(), P, S, OS, g.wplane_half_host, g.wtaps_f, g.wtaps_d,
-                        g.wplane_off, g.wplane_half, g.wplane_norm, g.w_nf, g.w_fbase))
+                        g.wplane_off, g.wplane_half, g.wplane_norm, g.wctaps_f, g.wctaps_d,
+                        g.wcplane_off, g.w_nf, g.w_fbase))
             return 1;
     }
     HIP_TRY(hipStreamSynchronize(g.stream));
