@@ -831,12 +831,7 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
 };
 
 template <int MODE, int PATH, int M, bool BATCH>
-#ifdef WCOL_WPE
-#define WCOL_ATTR __attribute__((amdgpu_waves_per_eu(WCOL_WPE, 8)))
-#else
-#define WCOL_ATTR
-#endif
-__global__ __launch_bounds__(WCOL_BLOCK) WCOL_ATTR void wcol_kernel(WArgs w) {
+__global__ __launch_bounds__(WCOL_BLOCK) void wcol_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
     typedef WColGridder<MODE, PATH, M, BATCH> Gd;
     Gd::run(w, smem + (threadIdx.x >> 6) * Gd::WAVE_BYTES);

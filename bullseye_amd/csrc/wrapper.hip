@@ -34,7 +34,8 @@ struct EventPair {
 };
 
 struct TimedCall {
-    hipEvent_t start, copied, k0, k1, stop;
+    // start/stop on the stream; cstart/copied around the staging copies (the copy stream)
+    hipEvent_t start, cstart, copied, k0, k1, stop;
     bool has_kernel;
     bool recorded;   // every event recorded (a call that failed midway is only released)
 };
@@ -60,8 +61,19 @@ struct State {
     // device buffers
     DevBuf grid, psfgrid, norm, counters, errflag, conv_f, conv_d, tapsum, geom, taper;
     DevBuf perm, blstart, keys, vals, keys_out, counts, ws, rowrec, chanrec;
-    DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_time,
-        in_jones, in_wl, in_enabled, in_cgi, in_fc;
+    // Host-buffer mode: two sets of staging buffers used by alternate calls.  A call's copies
+    // run on the copy stream once the call before the previous one (the last reader of the
+    // set) has finished on the stream, so they overlap the previous call's kernels; the
+    // stream waits for the copies before its first reader.
+    struct InputSet {
+        DevBuf vis, uvw, w, flags, rowflags, field, spw, a1, a2, time, jones, wl, enabled, cgi,
+            fc;
+        hipEvent_t staged = nullptr, done = nullptr;
+        bool used = false;
+    };
+    InputSet ins[2];
+    int ins_cur = 0;
+    hipStream_t cstream = nullptr;
     DevBuf out_tmp, fft_scratch, fft_real, jinv, walk, dg_model, dg_out, stokes_tmp, dft_smp,
         dft_comp;
     // w-projection: taps re-laid per (plane, frac_v, frac_u), per-plane supports and sums
@@ -127,7 +139,7 @@ static hipError_t table(DevBuf &b, size_t bytes, void **out) {
 
 // Copy `bytes` from the caller's array to a device staging buffer (host mode) or use it as
 // is (device-resident mode).
-hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
+hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out, hipStream_t st) {
     if (!src) {
         *out = nullptr;
         return hipSuccess;
@@ -138,9 +150,13 @@ hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
     }
     hipError_t e = b.ensure(bytes);
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, g.stream);
+    e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st);
     *out = b.p;
     return e;
+}
+
+hipError_t stage(DevBuf &b, const void *src, size_t bytes, const void **out) {
+    return stage(b, src, bytes, out, g.stream);
 }
 
 // Image-plane response of the AA kernel along one axis, normalised to 1 at the centre
@@ -192,14 +208,14 @@ static int barrier_impl() {
     for (auto &c : g.pending) {
         float ms = 0, kms = 0;
         if (!c.recorded) {
-            for (hipEvent_t ev : {c.start, c.copied, c.k0, c.k1, c.stop})
+            for (hipEvent_t ev : {c.start, c.cstart, c.copied, c.k0, c.k1, c.stop})
                 if (ev) (void)hipEventDestroy(ev);
             continue;
         }
         (void)hipEventElapsedTime(&ms, c.start, c.stop);
         g.grid_seconds += ms * 1e-3;
         float cms = 0;
-        (void)hipEventElapsedTime(&cms, c.start, c.copied);
+        (void)hipEventElapsedTime(&cms, c.cstart, c.copied);
         g.stats.copy_ms = dev_mode() ? 0.0 : cms;
         if (c.has_kernel) {
             (void)hipEventElapsedTime(&kms, c.k0, c.k1);
@@ -209,6 +225,7 @@ static int barrier_impl() {
             g.stats.prep_ms = pms;
         }
         (void)hipEventDestroy(c.start);
+        (void)hipEventDestroy(c.cstart);
         (void)hipEventDestroy(c.copied);
         (void)hipEventDestroy(c.k0);
         (void)hipEventDestroy(c.k1);
@@ -302,51 +319,74 @@ static void choose_streams(const gridding_parameters *p, size_t F, size_t B, siz
 // Inputs of a gridding or degridding call on the device (staged copies, or the caller's own
 // device pointers), the inverted Jones terms, the per-facet geometry and the scalar fields of
 // GridArgs that do not depend on the stream decomposition.
-// `copied` (optional) is recorded on the stream once the staging copies are queued.
+// `cstart` and `copied` (optional) are recorded around the staging copies.
 static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase, bool psf,
-                        bool need_vis, GridArgs &a, hipEvent_t copied = nullptr) {
+                        bool need_vis, GridArgs &a, hipEvent_t cstart = nullptr,
+                        hipEvent_t copied = nullptr) {
     const size_t rows = p->row_count, C = p->channel_count, npol = p->number_of_polarization_terms;
     const size_t nspw = std::max<size_t>(1, p->spw_count);
+    // staging set of this call (host mode): everything queued on the stream so far is the
+    // other set's last reader; this set's last reader was the call before that
+    hipStream_t cs = g.stream;
+    State::InputSet &in = g.ins[g.ins_cur ^ 1];
+    if (!dev_mode()) {
+        HIP_TRY(hipEventRecord(g.ins[g.ins_cur].done, g.stream));
+        g.ins[g.ins_cur].used = true;
+        g.ins_cur ^= 1;
+        cs = g.cstream;
+        if (in.used) HIP_TRY(hipStreamWaitEvent(cs, in.done, 0));
+    }
+    if (cstart) HIP_TRY(hipEventRecord(cstart, cs));
     const void *d;
-    HIP_TRY(stage(g.in_vis, (psf || !need_vis) ? nullptr : p->visibilities,
-                  rows * C * npol * 2 * sizeof(visibility_base_type), &d));
+    HIP_TRY(stage(in.vis, (psf || !need_vis) ? nullptr : p->visibilities,
+                  rows * C * npol * 2 * sizeof(visibility_base_type), &d, cs));
     a.vis = (const visibility_base_type *)d;
-    HIP_TRY(stage(g.in_uvw, p->uvw_coords, rows * 3 * sizeof(uvw_base_type), &d));
+    HIP_TRY(stage(in.uvw, p->uvw_coords, rows * 3 * sizeof(uvw_base_type), &d, cs));
     a.uvw = (const uvw_base_type *)d;
-    HIP_TRY(stage(g.in_wl, p->reference_wavelengths,
-                  nspw * C * sizeof(reference_wavelengths_base_type), &d));
+    HIP_TRY(stage(in.wl, p->reference_wavelengths,
+                  nspw * C * sizeof(reference_wavelengths_base_type), &d, cs));
     a.wavelengths = (const reference_wavelengths_base_type *)d;
-    HIP_TRY(stage(g.in_w, psf ? nullptr : p->visibility_weights,
-                  rows * C * npol * sizeof(visibility_weights_base_type), &d));
+    HIP_TRY(stage(in.w, psf ? nullptr : p->visibility_weights,
+                  rows * C * npol * sizeof(visibility_weights_base_type), &d, cs));
     a.weights = (const visibility_weights_base_type *)d;
-    HIP_TRY(stage(g.in_flags, psf ? nullptr : p->flags, rows * C * npol, &d));
+    HIP_TRY(stage(in.flags, psf ? nullptr : p->flags, rows * C * npol, &d, cs));
     a.flags = (const uint8_t *)d;
-    HIP_TRY(stage(g.in_rowflags, p->flagged_rows, rows, &d));
+    HIP_TRY(stage(in.rowflags, p->flagged_rows, rows, &d, cs));
     a.row_flags = (const uint8_t *)d;
-    HIP_TRY(stage(g.in_field, p->field_array, rows * 4, &d));
+    HIP_TRY(stage(in.field, p->field_array, rows * 4, &d, cs));
     a.field = (const uint32_t *)d;
-    HIP_TRY(stage(g.in_spw, p->spw_index_array, rows * 4, &d));
+    HIP_TRY(stage(in.spw, p->spw_index_array, rows * 4, &d, cs));
     a.spw = (const uint32_t *)d;
-    HIP_TRY(stage(g.in_a1, p->antenna_1_ids, rows * 4, &d));
+    HIP_TRY(stage(in.a1, p->antenna_1_ids, rows * 4, &d, cs));
     a.ant1 = (const uint32_t *)d;
-    HIP_TRY(stage(g.in_a2, p->antenna_2_ids, rows * 4, &d));
+    HIP_TRY(stage(in.a2, p->antenna_2_ids, rows * 4, &d, cs));
     a.ant2 = (const uint32_t *)d;
-    HIP_TRY(stage(g.in_enabled, p->enabled_channels, nspw * C, &d));
+    HIP_TRY(stage(in.enabled, p->enabled_channels, nspw * C, &d, cs));
     a.enabled = (const uint8_t *)d;
-    HIP_TRY(stage(g.in_cgi, psf ? p->sampling_function_channel_grid_indicies
-                                : p->channel_grid_indicies,
-                  nspw * C * sizeof(size_t), &d));
+    HIP_TRY(stage(in.cgi, psf ? p->sampling_function_channel_grid_indicies
+                              : p->channel_grid_indicies,
+                  nspw * C * sizeof(size_t), &d, cs));
     a.cgi = (const uint64_t *)d;
     // per-row arrays may be NULL in an empty chunk (the reference's loops then never run)
     const bool any_rows = rows > 0;
     const size_t nj = p->no_timestamps_read * p->antenna_count * g.F * nspw * C;
     if (mode == MODE_4J && any_rows) {
-        HIP_TRY(stage(g.in_time, p->timestamp_ids, rows * sizeof(size_t), &d));
+        HIP_TRY(stage(in.time, p->timestamp_ids, rows * sizeof(size_t), &d, cs));
         a.time_ids = (const uint64_t *)d;
-        HIP_TRY(stage(g.in_jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d));
+        HIP_TRY(stage(in.jones, p->jones_terms, nj * 8 * sizeof(visibility_base_type), &d, cs));
         a.jones = (const visibility_base_type *)d;
     }
-    if (copied) HIP_TRY(hipEventRecord(copied, g.stream));
+    // facet centres (facet_* modes) with the other inputs
+    const void *fc = nullptr;
+    if (facet) {
+        if (!p->facet_centres) return fail("facet_centres is NULL");
+        HIP_TRY(stage(in.fc, p->facet_centres, g.F * 2 * sizeof(uvw_base_type), &fc, cs));
+    }
+    if (copied) HIP_TRY(hipEventRecord(copied, cs));
+    if (!dev_mode()) {
+        HIP_TRY(hipEventRecord(in.staged, cs));
+        HIP_TRY(hipStreamWaitEvent(g.stream, in.staged, 0));
+    }
     if (!a.wavelengths || !a.enabled || !a.cgi ||
         (any_rows && (!a.uvw || !a.row_flags || !a.field || !a.spw ||
                       (!psf && ((need_vis && !a.vis) || !a.weights || !a.flags)))))
@@ -363,11 +403,6 @@ static int setup_inputs(gridding_parameters *p, int mode, bool facet, bool phase
 
     // per-facet geometry in FP64 (facet_* modes; grid_* modes use the phase centre), computed
     // on the device from the staged facet centres: no host buffer outlives the call
-    const void *fc = nullptr;
-    if (facet) {
-        if (!p->facet_centres) return fail("facet_centres is NULL");
-        HIP_TRY(stage(g.in_fc, p->facet_centres, g.F * 2 * sizeof(uvw_base_type), &fc));
-    }
     void *geom = nullptr;
     HIP_TRY(table(g.geom, g.F * sizeof(FacetGeom), &geom));
     HIP_TRY(launch_facet_geom((const uvw_base_type *)fc, (double)p->phase_centre_ra,
@@ -482,6 +517,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
         g.pending.push_back(TimedCall{});
         tcp = &g.pending.back();
         HIP_TRY(hipEventCreate(&tcp->start));
+        HIP_TRY(hipEventCreate(&tcp->cstart));
         HIP_TRY(hipEventCreate(&tcp->copied));
         HIP_TRY(hipEventCreate(&tcp->k0));
         HIP_TRY(hipEventCreate(&tcp->k1));
@@ -491,7 +527,7 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
     TimedCall &tc = *tcp;
 
     GridArgs a{};
-    if (setup_inputs(p, mode, facet, phase, psf, true, a, tc.copied)) return 1;
+    if (setup_inputs(p, mode, facet, phase, psf, true, a, tc.cstart, tc.copied)) return 1;
 
     // ingest: stable sort of rows by baseline
     void *perm, *keys, *vals, *keys_out, *counts, *blstart, *ws;
@@ -629,10 +665,6 @@ static int grid_call(gridding_parameters *p, int mode, bool facet, bool phase,
         tc.has_kernel = true;
         HIP_TRY(hipEventRecord(tc.stop, g.stream));
         tc.recorded = true;
-    }
-    if (!dev_mode()) {
-        // the caller may reuse its host arrays after gridding_barrier; staging copies are
-        // complete once the stream reaches the kernel
     }
     if (p->is_final_data_chunk) {
         if (barrier_impl()) return 1;
@@ -841,14 +873,23 @@ static void release_resources() {
                       &g.conv_d, &g.tapsum,
                       &g.geom, &g.taper, &g.perm, &g.blstart, &g.keys, &g.vals, &g.keys_out,
                       &g.rowrec, &g.chanrec,
-                      &g.counts, &g.ws, &g.in_vis, &g.in_uvw, &g.in_w, &g.in_flags,
-                      &g.in_rowflags, &g.in_field, &g.in_spw, &g.in_a1, &g.in_a2, &g.in_time,
-                      &g.in_jones, &g.in_wl, &g.in_enabled, &g.in_cgi, &g.in_fc, &g.out_tmp,
+                      &g.counts, &g.ws, &g.out_tmp,
                       &g.fft_scratch, &g.fft_real, &g.jinv, &g.wtaps_f, &g.wtaps_d,
                       &g.wplane_off, &g.wplane_half, &g.wplane_norm, &g.wcls, &g.wlist, &g.walk,
                       &g.wctaps_f, &g.wctaps_d, &g.wcplane_off,
                       &g.dg_model, &g.dg_out, &g.stokes_tmp, &g.dft_smp, &g.dft_comp};
     for (DevBuf *b : bufs) b->release();
+    for (State::InputSet &s : g.ins) {
+        DevBuf *ib[] = {&s.vis, &s.uvw, &s.w, &s.flags, &s.rowflags, &s.field, &s.spw, &s.a1,
+                        &s.a2, &s.time, &s.jones, &s.wl, &s.enabled, &s.cgi, &s.fc};
+        for (DevBuf *b : ib) b->release();
+        if (s.staged) (void)hipEventDestroy(s.staged);
+        if (s.done) (void)hipEventDestroy(s.done);
+        s.staged = s.done = nullptr;
+        s.used = false;
+    }
+    if (g.cstream) (void)hipStreamDestroy(g.cstream);
+    g.cstream = nullptr;
     batch_release_all();
     for (int i = 0; i < State::WSIDE; ++i) {
         if (g.wside[i]) (void)hipStreamDestroy(g.wside[i]);
@@ -886,6 +927,13 @@ static int init_impl(gridding_parameters *p) {
     }
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&g.wfork, hipEventDisableTiming));
+    HIP_TRY(hipStreamCreateWithFlags(&g.cstream, hipStreamNonBlocking));
+    for (State::InputSet &s : g.ins) {
+        HIP_TRY(hipEventCreateWithFlags(&s.staged, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        s.used = false;
+    }
+    g.ins_cur = 0;
     for (int i = 0; i < State::WSIDE; ++i) {
         HIP_TRY(hipStreamCreateWithFlags(&g.wside[i], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&g.wjoin[i], hipEventDisableTiming));

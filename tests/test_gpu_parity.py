@@ -107,6 +107,57 @@ def test_cube_channels_and_chunks():
     assert rms <= 2e-7 and peak <= 1e-6, (rms, peak)
 
 
+def time_chunks(obs, spec, k):
+    """k host-buffer chunks of whole timesteps (rows are time-ordered), each with the Jones
+    terms and rebased timestamp ids of its own timesteps, as the pipeliner hands them over."""
+    t = obs.time_idx.astype(np.int64)
+    bounds = np.searchsorted(t, np.linspace(0, obs.n_time, k + 1).round().astype(np.int64))
+    out = []
+    for i in range(k):
+        r0, r1 = int(bounds[i]), int(bounds[i + 1])
+        b = Binding("mixed", obs, spec, rows=(r0, r1))
+        if spec.jones is not None:
+            t0, t1 = int(t[r0]), int(t[r1 - 1]) + 1
+            b._set("timestamp_ids", (t[r0:r1] - t0).astype(np.uint64), np.uint64)
+            b._set("jones_terms", spec.jones[t0:t1], b.bt.visibility_type)
+            b.params.no_timestamps_read = t1 - t0
+        b.params.is_final_data_chunk = i == k - 1
+        out.append(b)
+    return out
+
+
+@pytest.mark.parametrize("mode", ["facet_4_cor_corrections", "grid_single_pol"])
+def test_host_chunks_back_to_back(mode):
+    """Host buffers in five chunks called back to back, no barrier between the calls: each
+    call stages its inputs on the copy stream into the staging set that the call before the
+    previous one read, while the previous call's kernels run.  The accumulated grid equals
+    the oracle's over all rows."""
+    print("case host chunks back to back", mode, flush=True)
+    obs = round_to(small_obs(mode, seed=21, n_time=25), "mixed")
+    spec = make_spec(obs, mode, precision="mixed")
+    want, want_norm = oracle_grid(obs, spec, mode)
+    lib = load_library("GPU", "mixed")
+    set_options(lib, compute_path=PATH_FP64)
+    chunks = time_chunks(obs, spec, 5)
+    assert lib.initLibrary(chunks[0].byref()) == 0, lib.bullseye_last_error()
+    for bc in chunks[1:]:   # one accumulation: the normalisation array initLibrary published
+        bc.params.normalization_terms = chunks[0].params.normalization_terms
+        bc.params.output_buffer = chunks[0].params.output_buffer
+    try:
+        for bc in chunks:
+            assert getattr(lib, mode)(bc.byref()) == 0, lib.bullseye_last_error()
+        lib.gridding_barrier()
+        norm = np.ctypeslib.as_array(
+            ctypes.cast(chunks[0].params.normalization_terms, ctypes.POINTER(ctypes.c_double)),
+            (len(want_norm),)).copy()
+        assert lib.normalize(chunks[-1].byref()) == 0
+    finally:
+        lib.releaseLibrary()
+    rms, peak = rel_err(chunks[0].output, want)
+    assert rms <= 2e-7 and peak <= 1e-6, (rms, peak)
+    np.testing.assert_allclose(norm, want_norm, rtol=1e-9)
+
+
 def test_singular_jones_reports_error():
     print("case singular jones", flush=True)
     mode = "facet_4_cor_corrections"
