@@ -55,6 +55,8 @@ def parse():
                          "2048^2 facet model through gridding and its degridding adjoint; "
                          "batch: the w-projection visibilities into 200 uneven facets in one "
                          "persistent graph launch")
+    ap.add_argument("--no-host", action="store_true",
+                    help="faceting workload: skip the host-buffer passes (profiling runs)")
     ap.add_argument("--no-batch", action="store_true",
                     help="faceting: leave out the batched-facets figures of the default line")
     ap.add_argument("--batch-facets", type=int, default=200)
@@ -298,45 +300,48 @@ def main():
     err_gpu = facet_error_table(grid32.reshape(F, 4, npx, npx), grid64.reshape(F, 4, npx, npx),
                                 norm32, norm64, taper)
     del grid64
-    # the reference's model: host arrays handed over every call (inputs copied to the device,
-    # grids copied back on the final chunk); reported beside `value`, never as it
-    bh = Binding("mixed", obs, spec)
-    o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=0,
-                         compute_path=PATH_FP32_KAHAN)
-    check(lib.bullseye_set_options(ctypes.byref(o)), "options")
-    check(lib.initLibrary(bh.byref()), "initLibrary(host)")
-    host_ms = []
-    for i in range(3):
-        t = time.perf_counter()
-        check(lib.bullseye_reset_grids(), "reset")
-        check(lib.facet_4_cor_corrections(bh.byref()), "grid(host)")
-        lib.gridding_barrier()
-        host_ms.append((time.perf_counter() - t) * 1e3)
-    lib.releaseLibrary()
-    del bh
-    wall_host = min(host_ms[1:]) * 1e-3
-    log("FP32-comp with host buffers (PCIe both ways): %.2f ms/step" % (wall_host * 1e3))
-    # the same pass as the pipeliner runs it: HOST_CHUNKS calls of whole timesteps, each
-    # call's staging copies overlapping the previous call's kernels (two staging sets)
-    chunks = host_chunk_bindings(obs, spec, HOST_CHUNKS)
-    check(lib.bullseye_set_options(ctypes.byref(o)), "options")
-    check(lib.initLibrary(chunks[-1].byref()), "initLibrary(host chunks)")
-    chunk_ms = []
-    for i in range(3):
-        t = time.perf_counter()
-        check(lib.bullseye_reset_grids(), "reset")
-        for bc in chunks:
-            check(lib.facet_4_cor_corrections(bc.byref()), "grid(host chunk)")
-        lib.gridding_barrier()
-        chunk_ms.append((time.perf_counter() - t) * 1e3)
-    lib.releaseLibrary()
-    grid_chunks = chunks[-1].host("output_buffer").astype(np.complex128)
-    del chunks
-    wall_chunks = min(chunk_ms[1:]) * 1e-3
-    chunk_err = rel_errors(grid_chunks.reshape(F, -1), grid32.reshape(F, -1))
-    del grid_chunks
-    log("FP32-comp with host buffers in %d chunks: %.2f ms/step (grid vs one call: rms %.1e)"
-        % (HOST_CHUNKS, wall_chunks * 1e3, chunk_err[0]))
+    wall_host = wall_chunks = None
+    chunk_err = (None, None)
+    if not args.no_host:
+        # the reference's model: host arrays handed over every call (inputs copied to the device,
+        # grids copied back on the final chunk); reported beside `value`, never as it
+        bh = Binding("mixed", obs, spec)
+        o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=0,
+                             compute_path=PATH_FP32_KAHAN)
+        check(lib.bullseye_set_options(ctypes.byref(o)), "options")
+        check(lib.initLibrary(bh.byref()), "initLibrary(host)")
+        host_ms = []
+        for i in range(3):
+            t = time.perf_counter()
+            check(lib.bullseye_reset_grids(), "reset")
+            check(lib.facet_4_cor_corrections(bh.byref()), "grid(host)")
+            lib.gridding_barrier()
+            host_ms.append((time.perf_counter() - t) * 1e3)
+        lib.releaseLibrary()
+        del bh
+        wall_host = min(host_ms[1:]) * 1e-3
+        log("FP32-comp with host buffers (PCIe both ways): %.2f ms/step" % (wall_host * 1e3))
+        # the same pass as the pipeliner runs it: HOST_CHUNKS calls of whole timesteps, each
+        # call's staging copies overlapping the previous call's kernels (two staging sets)
+        chunks = host_chunk_bindings(obs, spec, HOST_CHUNKS)
+        check(lib.bullseye_set_options(ctypes.byref(o)), "options")
+        check(lib.initLibrary(chunks[-1].byref()), "initLibrary(host chunks)")
+        chunk_ms = []
+        for i in range(3):
+            t = time.perf_counter()
+            check(lib.bullseye_reset_grids(), "reset")
+            for bc in chunks:
+                check(lib.facet_4_cor_corrections(bc.byref()), "grid(host chunk)")
+            lib.gridding_barrier()
+            chunk_ms.append((time.perf_counter() - t) * 1e3)
+        lib.releaseLibrary()
+        grid_chunks = chunks[-1].host("output_buffer").astype(np.complex128)
+        del chunks
+        wall_chunks = min(chunk_ms[1:]) * 1e-3
+        chunk_err = rel_errors(grid_chunks.reshape(F, -1), grid32.reshape(F, -1))
+        del grid_chunks
+        log("FP32-comp with host buffers in %d chunks: %.2f ms/step (grid vs one call: rms %.1e)"
+            % (HOST_CHUNKS, wall_chunks * 1e3, chunk_err[0]))
 
     gtaps32 = taps / wall32 / 1e9
     gtaps64 = taps64 / wall64 / 1e9
@@ -355,11 +360,11 @@ def main():
                    "facets": im["facets"], "npix": im["npix"], "taps_per_step": taps,
                    "samples_per_step": nsamp},
         "fp64_gtaps_per_s": round(gtaps64, 3), "fp32comp_over_fp64": round(gtaps32 / gtaps64, 3),
-        "host_buffers_ms_per_step": round(wall_host * 1e3, 3),
-        "host_buffers_gtaps_per_s": round(taps / wall_host / 1e9, 3),
-        "host_buffers_chunked": {"chunks": HOST_CHUNKS, "ms_per_step": round(wall_chunks * 1e3, 3),
-                                 "gtaps_per_s": round(taps / wall_chunks / 1e9, 3),
-                                 "rms_vs_one_call": chunk_err[0]},
+        "host_buffers_ms_per_step": None if wall_host is None else round(wall_host * 1e3, 3),
+        "host_buffers_gtaps_per_s": None if wall_host is None else round(taps / wall_host / 1e9, 3),
+        "host_buffers_chunked": None if wall_chunks is None else {
+            "chunks": HOST_CHUNKS, "ms_per_step": round(wall_chunks * 1e3, 3),
+            "gtaps_per_s": round(taps / wall_chunks / 1e9, 3), "rms_vs_one_call": chunk_err[0]},
         "rms_err_vs_fp64_gpu": rms_vs_gpu64, "peak_err_vs_fp64_gpu": peak_vs_gpu64,
         "error_per_facet_vs_fp64_gpu": err_gpu,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3),
