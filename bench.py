@@ -597,7 +597,7 @@ def main_wproj(args):
 
     def set_opts(path):
         o = bullseye_options(struct_size=ctypes.sizeof(bullseye_options), device_resident=1,
-                             compute_path=path)
+                             compute_path=path, channels_per_stream=args.channels_per_stream)
         check(lib.bullseye_set_options(ctypes.byref(o)), "options")
 
     set_opts(PATH_FP32_KAHAN)
