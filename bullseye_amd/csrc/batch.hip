@@ -3,12 +3,12 @@
 //
 // The reference images one facet geometry per initLibrary and its pipeliner loops over calls
 // (bullseye/bin/bullseye_pipeliner.py:350-404); every facet is a separate pass over the
-// visibilities.  Here a plan holds up to 255 facets over one staged, baseline-sorted visibility
-// set:
+// visibilities.  Here a plan holds up to 65536 facets over one staged, baseline-sorted
+// visibility set:
 //   * every kernel stack ("kernel set") classifies the streams -- (baseline, channel block,
 //     segment of base_rows rows) -- by the widest w-plane any of their samples uses
 //     (k_stream_class, wproj.hip); facets sharing a kernel set share its classes;
-//   * per (kind, class width M) one work list holds the entries (facet << 24 | stream) of every
+//   * per (kind, class width M) one work list holds the entries (facet << 32 | stream) of every
 //     facet, streams in order and the facets of a stream together, so the visibilities a
 //     stream reads are reused from L2 by all facets in turn;
 //   * one persistent launch per list drains it through an atomic work counter (the class
@@ -33,10 +33,10 @@ namespace bam {
 namespace {
 
 constexpr uint32_t MAX_SETS = 64;
-constexpr uint32_t MAX_FACETS = 255;
+constexpr uint32_t MAX_FACETS = 1u << 16;
 constexpr int BRANCHES = 4;
 constexpr uint32_t NCLS = 32;   // class widths 1, 3, ..., 63
-constexpr uint32_t LIST_PAD = 0xffffffu;   // stream id of a no-op work-list entry
+constexpr uint32_t LIST_PAD = 0xffffffffu;   // stream id of a no-op work-list entry
 
 // Streams per column-kernel item of class width M (1 where the class has no column kernel).
 uint32_t column_run(int M, int kind, int path) {
@@ -84,14 +84,15 @@ struct Plan {
     std::vector<size_t> acc_off;             // doubles
     size_t acc_elems = 0, norm_stride = 0;
     std::vector<uint32_t> set_slot;          // kernel set id -> row of cls_all (~0: unused)
-    DevBuf acc, norm, fargs, geom, lists, queues, cls_all, out_tmp, walk;
+    DevBuf acc, norm, fargs, geom, lists, lists64, queues, cls_all, out_tmp, walk;
     DevBuf in_vis, in_uvw, in_w, in_flags, in_rowflags, in_field, in_spw, in_a1, in_a2, in_wl,
         in_enabled, in_cgi, in_sfcgi;
     DevBuf perm, keys, vals, keys_out, counts, blstart, ws, rowrec;
     std::vector<Launch> launches;
     std::vector<wp::WArgs> fargs_host;
     std::vector<FacetGeom> geom_host;
-    std::vector<uint32_t> lists_host;
+    std::vector<uint32_t> lists_host;        // per-facet launches: plain stream ids
+    std::vector<uint64_t> lists64_host;      // merged launches: facet << 32 | stream
     std::vector<uint8_t> cls_host;
     std::vector<double> norm_host;
     GridArgs base{};
@@ -108,7 +109,8 @@ struct Plan {
     }
     void release() {
         drop_graph();
-        DevBuf *bufs[] = {&acc, &norm, &fargs, &geom, &lists, &queues, &cls_all, &out_tmp, &walk,
+        DevBuf *bufs[] = {&acc, &norm, &fargs, &geom, &lists, &lists64, &queues, &cls_all,
+                          &out_tmp, &walk,
                           &in_vis, &in_uvw, &in_w, &in_flags, &in_rowflags, &in_field, &in_spw,
                           &in_a1, &in_a2, &in_wl, &in_enabled, &in_cgi, &in_sfcgi, &perm, &rowrec, &keys,
                           &vals, &keys_out, &counts, &blstart, &ws};
@@ -126,6 +128,8 @@ struct Plan {
         launches.clear();
         lists_host.clear();
         lists_host.shrink_to_fit();
+        lists64_host.clear();
+        lists64_host.shrink_to_fit();
         ready = false;
     }
 };
@@ -246,6 +250,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
     const LibView lv = lib_view();
     pl.launches.clear();
     pl.lists_host.clear();
+    pl.lists64_host.clear();
     const uint32_t nsets_used = (uint32_t)(pl.cls_host.size() / std::max<uint64_t>(1, pl.nbase));
     std::vector<std::vector<std::vector<uint32_t>>> streams(nsets_used);
     for (uint32_t k = 0; k < nsets_used; ++k) class_streams(pl, k, t, streams[k]);
@@ -253,7 +258,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         const uint32_t chunk = chunk_of(t, 2 * i + 1);
         const uint64_t nsegk = (pl.nseg_base + chunk - 1) / chunk;
         if ((uint64_t)pl.B * pl.ncb * nsegk >= LIST_PAD)
-            return fail("batch too large: %llu streams per facet exceed 2^24",
+            return fail("batch too large: %llu streams per facet exceed 2^32 - 1",
                         (unsigned long long)((uint64_t)pl.B * pl.ncb * nsegk));
     }
     // facets of each (kernel set row, kind)
@@ -287,7 +292,11 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
         L.w.nseg_base = pl.nseg_base;
         L.w.filter = chunk > 1 ? 1 : 0;
         L.w.M = M;
-        L.w.stream_list = (const uint32_t *)(uintptr_t)off;  // offset until the upload
+        // offsets into the plain (per-facet) or the merged lists until the upload
+        if (facet >= 0)
+            L.w.stream_list = (const uint32_t *)(uintptr_t)off;
+        else
+            L.w.stream_list64 = (const uint64_t *)(uintptr_t)off;
         L.w.n_list = (uint32_t)cnt;
         L.mode = kind;
         L.M = M;
@@ -304,8 +313,8 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
     // facet, so the kernel reads that facet's arguments once per wave with scalar loads; a
     // short run is padded with no-op entries (stream PAD).
     auto emit_run = [&](uint32_t f, const uint32_t *ids, size_t n, uint32_t g) {
-        for (size_t q = 0; q < n; ++q) pl.lists_host.push_back((f << 24) | ids[q]);
-        for (size_t q = n; q % g; ++q) pl.lists_host.push_back((f << 24) | LIST_PAD);
+        for (size_t q = 0; q < n; ++q) pl.lists64_host.push_back((uint64_t)f << 32 | ids[q]);
+        for (size_t q = n; q % g; ++q) pl.lists64_host.push_back((uint64_t)f << 32 | LIST_PAD);
     };
     for (int kind : {pl.mode, (int)MODE_PSF}) {
         auto &of = kind == MODE_PSF ? of_psf : of_vis;
@@ -323,7 +332,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
             continue;
         }
         for (uint32_t i = 0; i < NCLS; ++i) {
-            const uint64_t off = pl.lists_host.size();
+            const uint64_t off = pl.lists64_host.size();
             const uint32_t g = column_run((int)(2 * i + 1), kind, pl.path);
             if (g > 1 && !t.facet_major) {
                 // runs of g consecutive streams of a kernel set, every facet of the set in turn
@@ -341,7 +350,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     const size_t n = std::min<size_t>(g, sv.size() - q0);
                     for (uint32_t f : of[r.second]) emit_run(f, sv.data() + q0, n, g);
                 }
-                if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
+                if (add_launch(kind, i, off, pl.lists64_host.size() - off, -1)) return 1;
                 continue;
             }
             if (t.facet_major) {  // each facet's streams together
@@ -350,7 +359,7 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     const std::vector<uint32_t> &sv = streams[pl.set_slot[pl.fset[f]]][i];
                     emit_run(f, sv.data(), sv.size(), g);
                 }
-                if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
+                if (add_launch(kind, i, off, pl.lists64_host.size() - off, -1)) return 1;
                 continue;
             }
             // merge the kernel sets' ascending stream lists; each stream's facets together
@@ -365,17 +374,21 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
                     if (of[k].empty() || pos[k] >= streams[k][i].size() ||
                         streams[k][i][pos[k]] != best)
                         continue;
-                    for (uint32_t f : of[k]) pl.lists_host.push_back((f << 24) | best);
+                    for (uint32_t f : of[k]) pl.lists64_host.push_back((uint64_t)f << 32 | best);
                     ++pos[k];
                 }
             }
-            if (add_launch(kind, i, off, pl.lists_host.size() - off, -1)) return 1;
+            if (add_launch(kind, i, off, pl.lists64_host.size() - off, -1)) return 1;
         }
     }
     // device lists, queues and branch assignment (longest first onto the least loaded)
     HIP_TRY(pl.lists.ensure(std::max<size_t>(4, pl.lists_host.size() * 4)));
     if (!pl.lists_host.empty())
         HIP_TRY(hipMemcpyAsync(pl.lists.p, pl.lists_host.data(), pl.lists_host.size() * 4,
+                               hipMemcpyHostToDevice, lv.stream));
+    HIP_TRY(pl.lists64.ensure(std::max<size_t>(8, pl.lists64_host.size() * 8)));
+    if (!pl.lists64_host.empty())
+        HIP_TRY(hipMemcpyAsync(pl.lists64.p, pl.lists64_host.data(), pl.lists64_host.size() * 8,
                                hipMemcpyHostToDevice, lv.stream));
     HIP_TRY(pl.queues.ensure(std::max<size_t>(4, pl.launches.size() * 4)));
     std::vector<size_t> order(pl.launches.size());
@@ -393,12 +406,16 @@ int build_lists(Plan &pl, const bullseye_batch_tuning &t) {
     }
     for (size_t j = 0; j < pl.launches.size(); ++j) {
         Launch &L = pl.launches[j];
-        L.w.stream_list = pl.lists.as<uint32_t>() + (uintptr_t)L.w.stream_list;
+        if (L.single)
+            L.w.stream_list = pl.lists.as<uint32_t>() + (uintptr_t)L.w.stream_list;
+        else
+            L.w.stream_list64 = pl.lists64.as<uint64_t>() + (uintptr_t)L.w.stream_list64;
         L.w.facet_args = L.single ? nullptr : pl.fargs.as<wp::WArgs>();
         L.w.queue = per_facet ? nullptr : pl.queues.as<uint32_t>() + j;
     }
     HIP_TRY(hipStreamSynchronize(lv.stream));  // lists_host stays valid until the upload is done
     std::vector<uint32_t>().swap(pl.lists_host);
+    std::vector<uint64_t>().swap(pl.lists64_host);
     return 0;
 }
 

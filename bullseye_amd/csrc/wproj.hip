@@ -288,11 +288,10 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
             // a stream slot is whole wavefronts, so the entry is uniform across the wave
-            uint32_t e = live ? w.stream_list[li] : 0u;
-            e = __builtin_amdgcn_readfirstlane(e);
-            fidx = e >> 24;
-            stream = e & 0xffffffu;
-            if (stream == 0xffffffu) stream = 0, live = false;   // padding entry
+            const uint64_t e = live ? w.stream_list64[li] : 0ull;
+            fidx = __builtin_amdgcn_readfirstlane((uint32_t)(e >> 32));
+            stream = __builtin_amdgcn_readfirstlane((uint32_t)e);
+            if (stream == 0xffffffffu) stream = 0, live = false;   // padding entry
             wfp = w.facet_args + fidx;
         }
         const WArgs &wf = *wfp;
@@ -697,9 +696,10 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
         const WArgs *wfp = &w;
         if constexpr (BATCH) {
             // the host lays out an item's GPW entries with one facet: read it once per wave
-            fidx = __builtin_amdgcn_readfirstlane(w.stream_list[it * (uint64_t)GPW] >> 24);
-            const uint32_t stream = w.stream_list[li] & 0xffffffu;
-            if (stream == 0xffffffu) return;   // padding
+            fidx = __builtin_amdgcn_readfirstlane(
+                (uint32_t)(w.stream_list64[it * (uint64_t)GPW] >> 32));
+            const uint32_t stream = (uint32_t)w.stream_list64[li];
+            if (stream == 0xffffffffu) return;   // padding
             wfp = w.facet_args + fidx;
             uint64_t t = stream;
             facet = (uint32_t)(t % wfp->g.facets);
@@ -1181,6 +1181,7 @@ hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void
     w.plane_half = plane_half;
     w.plane_norm = plane_norm;
     w.stream_list = reinterpret_cast<const uint32_t *>(b + pl.off_val_sorted);
+    w.stream_list64 = nullptr;
     w.n_list = 0;
     w.wplanes = wplanes;
     w.nf = nf;

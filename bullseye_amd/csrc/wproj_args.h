@@ -22,6 +22,8 @@ struct WArgs {
     const void *ctaps;                 // complex acc_t
     const uint64_t *cplane_off;
     const uint32_t *stream_list;       // streams of this class
+    // batched facets: entries (facet << 32 | stream), stream 0xffffffff a no-op padding entry
+    const uint64_t *stream_list64;
     uint32_t n_list;
     uint32_t wplanes, nf, fbase;       // fraction index range [fbase, fbase + nf)
     uint32_t M;                        // class width (odd)

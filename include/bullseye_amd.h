@@ -313,7 +313,7 @@ int bullseye_dft_predict(gridding_parameters *params, const double *components, 
 /* ---- batched facets (w-projection) ----------------------------------------------------------
  * The reference grids one facet geometry per initLibrary: every facet of a call shares nx, ny,
  * cell size and kernels (gridding_parameters.h:45-72), and the pipeliner loops over calls
- * (bullseye/bin/bullseye_pipeliner.py:350-404).  These entry points grid up to 255 facets of
+ * (bullseye/bin/bullseye_pipeliner.py:350-404).  These entry points grid up to 65536 facets of
  * different size, cell, w-kernel stack and kind (visibilities or sampling function) from one
  * visibility set in a single plan: the facets' streams are merged into one work queue per
  * support class and drained by persistent kernels, captured once in a HIP graph and replayed.

@@ -19,12 +19,12 @@ import torch
 from common import rel_err, round_to, set_options, small_obs, wproj_spec
 from bullseye_amd import synth
 from bullseye_amd.base_types import force_precision
-from bullseye_amd.batch import BatchFacet, FacetBatch
+from bullseye_amd.batch import GRID_MODES, BatchFacet, FacetBatch
 from bullseye_amd.binding import Binding
 from bullseye_amd.library_loader import (BATCH_PER_FACET, BATCH_PERSISTENT,
                                          BATCH_PERSISTENT_GRAPH, PATH_FP32_KAHAN, PATH_FP64,
                                          SHAPE_COLUMN, SHAPE_PAIRS1, SHAPE_PAIRS2, SHAPE_PAIRS4,
-                                         bullseye_stats, load_library)
+                                         bullseye_batch_facet, bullseye_stats, load_library)
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -405,24 +405,26 @@ def test_batch_non_square_facet_matches_oracle():
     np.testing.assert_allclose(norms[0].reshape(-1), want_norm, rtol=1e-9)
 
 
-def test_batch_holds_255_facets():
-    """The widest batch (facet index in 8 bits): 255 small facets of both kinds; every facet's
-    taps add up to those of its single-facet call (16 checked against the full grid), and a 256th
-    facet is refused."""
-    print("case batch 255 facets", flush=True)
+def test_batch_holds_300_facets():
+    """A batch wider than an 8-bit facet index: 300 small facets of both kinds; every facet's
+    taps add up to those of its single-facet call (one in 16 checked against the full grid), and
+    a plan of more than 65536 facets is refused before it allocates anything."""
+    print("case batch 300 facets", flush=True)
     precision, path = "double", PATH_FP64
     obs = observation(precision, seed=27)
     kspecs = kernel_specs(obs, precision)
     rng = np.random.default_rng(20)
     table = [(int(rng.choice([32, 48, 64])), float(rng.choice([20.0, 30.0, 40.0])),
-              int(rng.integers(0, 2)), bool(rng.random() < 0.25)) for _ in range(255)]
+              int(rng.integers(0, 2)), bool(rng.random() < 0.25)) for _ in range(300)]
     table = [(n, c, 0 if n < 40 else k, p) for n, c, k, p in table]  # set 1 needs > 2*12+4 px
     facets = facet_list(obs, precision, facets=table, seed=22)
     lib, s = open_session(precision, path, obs, kspecs)
     try:
         grids, norms, taps = s.run(facets)
-        with pytest.raises(RuntimeError, match="at most 255"):
-            s.batch.plan(facets + facets[:1])
+        too_many = (bullseye_batch_facet * 65537)()
+        assert lib.bullseye_batch_plan(s.batch.binding.byref(), GRID_MODES["facet_single_pol"],
+                                       too_many, 65537, 0) != 0
+        assert b"at most 65536" in lib.bullseye_last_error()
     finally:
         s.close()
     lib = load_library("GPU", precision)
