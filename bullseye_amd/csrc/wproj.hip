@@ -519,10 +519,11 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
     }
 
     // Move this lane's ownership to footprint (x0, y0, cgi): flush the points it gives up.
-    // Each row that leaves (every row when the slice changes or more than two columns leave),
-    // and then the other rows of each column that leaves, go through the group's LDS staging
+    // Rows that leave (and every row when the slice changes or the footprint jumps by more
+    // than two columns) are flushed by all lanes of the group, adjacent cells of a grid row
+    // together.  The other rows of a column that leaves go through the group's LDS staging
     // area, so that the real and imaginary parts (and the correlations) of a cell are added by
-    // adjacent lanes of one instruction instead of a request per value and lane.
+    // adjacent lanes of one instruction instead of one lane issuing a request per value.
     static __device__ __forceinline__ void move_to(const GridArgs &a, uint32_t facet, int gw,
                                                    int gl, int ua, int c0, int x0, int y0,
                                                    int cgi, int &px0, int &py0, int &pcgi,
@@ -533,38 +534,11 @@ template <int MODE, int PATH, int M, bool BATCH> struct WColGridder {
             const int xo = px0 + mod_m(ua - pxm);
             const int xn = x0 + mod_m(ua - x0 % M);
             const bool every = cgi != pcgi || x0 - px0 > 2 || px0 - x0 > 2;
-            // each row that leaves (every row when the slice changes or more than two
-            // columns leave) goes through the staging area: the group adds its M cells with
-            // real and imaginary parts (and correlations) in adjacent lanes
 #pragma unroll
             for (int r = 0; r < M; ++r) {
                 const int yo = py0 + mod_m(r - pym);
-                const bool leave = every || y0 + mod_m(r - ym) != yo;   // the same for the group
-                if (!__ballot(leave)) continue;
-                if (leave) {
-#pragma unroll
-                    for (int k = 0; k < CPL; ++k) {
-                        stage[(ua * NC + c0 + k) * 2] = acc[r][k][0].value();
-                        stage[(ua * NC + c0 + k) * 2 + 1] = acc[r][k][1].value();
-                        acc[r][k][0].zero();
-                        acc[r][k][1].zero();
-                    }
-                }
-                wave_sync();
-                if (leave) {
-                    const size_t row = (((size_t)facet * a.cube + pcgi) * a.ny + yo) * a.nx;
-#pragma unroll
-                    for (int i = 0; i < 2 * CPL; ++i) {
-                        const int j = gl + i * G;
-                        const double v = stage[j];
-                        if (v != 0.0) {
-                            const int x = px0 + mod_m(j / (2 * NC) - pxm);
-                            unsafeAtomicAdd(a.grid + ((row + x) * a.pstride) * 2 + j % (2 * NC),
-                                            v);
-                        }
-                    }
-                }
-                wave_sync();
+                const int yn = y0 + mod_m(r - ym);
+                if (every || yn != yo) flush1(a, facet, pcgi, xo, yo, c0, acc[r]);
             }
             bool pend = !every && xn != xo;
             for (;;) {
