@@ -19,7 +19,12 @@
 // stream's current footprint (Romein 2012); a sample narrower than M touches only the pairs
 // inside its own W_p window.  Small classes pack several streams into a 256-lane workgroup;
 // classes wider than 16 use 1024-lane workgroups with up to PAIRS residue pairs per lane.
+// The 1024-lane workgroups (one per CU) hold an on-chip FP64 window of the grid around each
+// stream's track in their spare LDS: the flushes that land in it are added there, and the
+// window goes to the grid once per stream, whole grid-row runs per atomic instruction (on the
+// 4096^2 w-projection workload 2.7x fewer flush requests from these kernels).
 #include <algorithm>
+#include <cmath>
 
 #include <hipcub/hipcub.hpp>
 
@@ -235,24 +240,106 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
     static constexpr bool KAHAN = PathTraits<PATH>::KAHAN;
     typedef WRec<T, NC> R_t;
 
+    // On-chip window of the grid around a stream's track.  Cell (x, y) is always owned by the
+    // residue pair (x mod M, y mod M), so each window cell has one owning lane: the lane keeps
+    // its cells in private LDS slots (slot (p, j, k) of lane sl at ((p * win^2 + j) * NC + k) * L
+    // + sl, j = jy * win + jx the cell's index among the pair's cells of the window) and adds to
+    // them without atomics, in FP64 like the grid.  The window is flushed to the grid once per
+    // stream, a row of cells per instruction.
+    struct Win {
+        cpx<double> *lane;   // slot (0, 0, 0) of this lane; nullptr: no window
+        int L, win, x0, y0, X, Y, cgi;
+        float inv_m;
+        // This lane's slot of correlation 0 for cell (x, y) of slice cgi, residue pair p; nullptr
+        // when the cell lies outside the window.
+        __device__ __forceinline__ cpx<double> *slot(int p, int c, int x, int y) const {
+            const int dx = x - x0, dy = y - y0;
+            if (!lane || c != cgi || (unsigned)dx >= (unsigned)X || (unsigned)dy >= (unsigned)Y)
+                return nullptr;
+            // dx = r + jx M with r < M: (dx + 1/2) / M lies at least 1/(2M) from an integer
+            const int jx = (int)(((float)dx + 0.5f) * inv_m);
+            const int jy = (int)(((float)dy + 0.5f) * inv_m);
+            return lane + ((p * win * win + jy * win + jx) * NC) * L;
+        }
+    };
+
     static __device__ __forceinline__ void flush(const GridArgs &a, uint32_t facet, int cgi,
-                                                 int x, int y, acc1<T, KAHAN> (&acc)[NC][2]) {
+                                                 int x, int y, acc1<T, KAHAN> (&acc)[NC][2],
+                                                 const Win &wn, int p) {
         bool nz = false;
 #pragma unroll
         for (int k = 0; k < NC; ++k) nz |= (acc[k][0].s != T(0)) | (acc[k][1].s != T(0));
         if (nz) {
-            double *cell = a.grid + ((((size_t)facet * a.cube + cgi) * a.ny + y) * a.nx + x) *
-                                        (size_t)a.pstride * 2;
+            cpx<double> *sl = wn.slot(p, cgi, x, y);
+            if (sl) {
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                unsafeAtomicAdd(cell + 2 * k, acc[k][0].value());
-                unsafeAtomicAdd(cell + 2 * k + 1, acc[k][1].value());
+                for (int k = 0; k < NC; ++k) {
+                    cpx<double> v = sl[k * wn.L];
+                    v.re += acc[k][0].value();
+                    v.im += acc[k][1].value();
+                    sl[k * wn.L] = v;
+                }
+            } else {
+                double *cell = a.grid + ((((size_t)facet * a.cube + cgi) * a.ny + y) * a.nx + x) *
+                                            (size_t)a.pstride * 2;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    unsafeAtomicAdd(cell + 2 * k, acc[k][0].value());
+                    unsafeAtomicAdd(cell + 2 * k + 1, acc[k][1].value());
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
             acc[k][0].zero();
             acc[k][1].zero();
+        }
+    }
+
+    // Adds the window's nonzero values into the grid: lane sl of the stream takes one double
+    // (correlation, re/im) of one cell per window row, R rows per pass, so an instruction adds
+    // whole runs of a grid row (one request per 64-byte segment).
+    static __device__ __forceinline__ void flush_window(const GridArgs &a, uint32_t facet,
+                                                        const Win &wn, const cpx<double> *wslot,
+                                                        int M, int sl) {
+        const int D = 2 * NC * wn.X;   // doubles per window row (the launch keeps D <= L)
+        const int R = wn.L / D;        // window rows per pass
+        const int r = sl / D;
+        if (r >= R) return;
+        const int e = sl - r * D;
+        const int cx = e / (2 * NC), kc = e - cx * (2 * NC);
+        const int k = kc >> 1, comp = kc & 1;
+        const int x = wn.x0 + cx;
+        const int jx = cx / M;
+        const int am = ((x % M) + M) % M;           // u residue of the column
+        const int ym = ((wn.y0 % M) + M) % M;       // v residue of the window's first row
+        const double *ws = reinterpret_cast<const double *>(wslot);
+        int jy = 0, rm = r;                         // window row ry = jy * M + rm
+        while (rm >= M) {
+            rm -= M;
+            ++jy;
+        }
+        for (int ry = r; ry < wn.Y; ry += R) {
+            int b = ym + rm;
+            if (b >= M) b -= M;
+            int pair = b * M + am, p = 0;           // owner: residue pair b * M + am
+            while (pair >= wn.L) {
+                pair -= wn.L;
+                ++p;
+            }
+            const double v =
+                ws[((size_t)((p * wn.win * wn.win + jy * wn.win + jx) * NC + k) * wn.L + pair) * 2 +
+                   comp];
+            if (v != 0.0)
+                unsafeAtomicAdd(a.grid + ((((size_t)facet * a.cube + wn.cgi) * a.ny +
+                                           (size_t)(wn.y0 + ry)) * a.nx + (size_t)x) *
+                                             (size_t)a.pstride * 2 + 2 * k + comp,
+                                v);
+            rm += R;
+            while (rm >= M) {
+                rm -= M;
+                ++jy;
+            }
         }
     }
 
@@ -352,6 +439,31 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                 acc[p][k][0].zero();
                 acc[p][k][1].zero();
             }
+        // the stream slot's window (placed after the first step's records are prepared)
+        Win wn;
+        wn.lane = nullptr;
+        wn.L = L;
+        wn.win = (int)w.win;
+        wn.X = wn.Y = wn.win * M;
+        wn.x0 = wn.y0 = 0;
+        wn.cgi = -1;
+        wn.inv_m = 1.0f / (float)M;
+        cpx<double> *wslot = nullptr;   // the slot's window area
+        if (wn.win) {
+            const size_t rec_bytes =
+                ((size_t)w.streams_per_block * nrec * sizeof(R_t) + 15) & ~(size_t)15;
+            wslot = reinterpret_cast<cpx<double> *>(smem + rec_bytes) +
+                    (size_t)slot * L * PAIRS * wn.win * wn.win * NC;
+            if (live) {
+                // each lane clears its own slots (the previous stream's window flush, which
+                // reads every lane's slots, ended before this item's first barrier)
+#pragma unroll
+                for (int p = 0; p < PAIRS; ++p)
+                    if (own[p])
+                        for (int j = 0; j < wn.win * wn.win * NC; ++j)
+                            wslot[(p * wn.win * wn.win * NC + j) * L + sl] = {0.0, 0.0};
+            }
+        }
         int ncgi = -1;
         double nacc[NC];
 #pragma unroll
@@ -361,8 +473,10 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         // normalisation key: facet slice, or the global facet of a batched entry
         const int kbase = BATCH ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
+        __shared__ int s_box[16][5];   // per wavefront: its records' footprint box and slice
         for (uint32_t st = 0; st < steps; ++st) {
             const uint32_t step0 = st * nrec;
+            int bx = 0, by = 0, bc = -1;    // this lane's record: footprint origin and slice
             if (live && sl < (int)nrec) {
                 R_t rec;
                 rec.cgi = -1;
@@ -391,8 +505,54 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                     }
                 }
                 recs[sl] = rec;
+                bx = rec.x0;
+                by = rec.y0;
+                bc = rec.cgi;
+            }
+            if (wn.win && st == 0) {
+                // box of the first step's footprints (the stream's first samples in walk order),
+                // reduced per wavefront
+                int lx = bc >= 0 ? bx : 0x7fffffff, hx = bc >= 0 ? bx : -0x7fffffff;
+                int ly = bc >= 0 ? by : 0x7fffffff, hy = bc >= 0 ? by : -0x7fffffff;
+                int lc = bc >= 0 ? bc : 0x7fffffff;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    lx = min(lx, __shfl_xor(lx, o));
+                    hx = max(hx, __shfl_xor(hx, o));
+                    ly = min(ly, __shfl_xor(ly, o));
+                    hy = max(hy, __shfl_xor(hy, o));
+                    lc = min(lc, __shfl_xor(lc, o));
+                }
+                if ((threadIdx.x & 63) == 0) {
+                    int *b = s_box[threadIdx.x >> 6];
+                    b[0] = lx;
+                    b[1] = hx;
+                    b[2] = ly;
+                    b[3] = hy;
+                    b[4] = lc;
+                }
             }
             __syncthreads();
+            if (wn.win && st == 0 && live) {
+                // the window is centred on the box of the slot's preparing wavefronts
+                int lx = 0x7fffffff, hx = -0x7fffffff, ly = 0x7fffffff, hy = -0x7fffffff;
+                int lc = 0x7fffffff;
+                const int w0 = slot * (L >> 6);
+                for (int i = 0; i < (int)(nrec >> 6); ++i) {
+                    const int *b = s_box[w0 + i];
+                    lx = min(lx, b[0]);
+                    hx = max(hx, b[1]);
+                    ly = min(ly, b[2]);
+                    hy = max(hy, b[3]);
+                    lc = min(lc, b[4]);
+                }
+                if (lc != 0x7fffffff) {
+                    wn.cgi = lc;
+                    wn.x0 = (lx + hx + M) / 2 - wn.X / 2;
+                    wn.y0 = (ly + hy + M) / 2 - wn.Y / 2;
+                    wn.lane = wslot + sl;
+                }
+            }
             const int kmax = live ? (int)min(nrec, nsamp > step0 ? nsamp - step0 : 0u) : 0;
             // records in chunks of UNR: the chunk's tap loads are issued together, so their
             // latency overlaps instead of one load in flight per lane at a time
@@ -424,7 +584,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                         const int kv = mod_m(pb[p] - r.ym, M);
                         const int x = r.x0 + ku, y = r.y0 + kv;
                         if (x != px[p] || y != py[p] || r.cgi != pc[p]) {
-                            if (pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
+                            if (pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p], wn, p);
                             px[p] = x;
                             py[p] = y;
                             pc[p] = r.cgi;
@@ -446,7 +606,11 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         }
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p)
-            if (own[p] && pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p]);
+            if (own[p] && pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p], wn, p);
+        if (wn.win) {
+            __syncthreads();   // every lane's sums are in the window
+            if (live && wn.lane) flush_window(a, facet, wn, wslot, M, sl);
+        }
         // every thread of the block gets here; sums go to the launch's slots (batched facets
         // share one normalisation array, keyed by global facet)
         if constexpr (MODE != MODE_PSF) publish_norm<NC>(w.g, ncgi >= 0 ? kbase + ncgi : -1, nacc);
@@ -892,15 +1056,52 @@ __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
     WGridder<MODE, PATH, PAIRS, BATCH>::run(w, smem);
 }
 
+// LDS a workgroup may allocate on this device.
+static size_t lds_per_block() {
+    static int bytes = 0;
+    if (!bytes) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess ||
+            bytes <= 0)
+            bytes = 65536;
+    }
+    return (size_t)bytes;
+}
+
+// Window size (cells per residue pair and axis) of a pair-kernel launch: the largest whose slots
+// fit the LDS left beside the records, at least 2, with a window row's doubles no more than a
+// stream's lanes (the window flush takes a row per lane set).  Only 1024-lane workgroups (one per
+// CU) take a window; smaller ones keep their LDS for more resident workgroups.
+static uint32_t window_cells(uint32_t block, uint32_t lanes, uint32_t spb, int pairs, int nc,
+                             uint32_t m_max, size_t rec_bytes) {
+    if (block != 1024) return 0;
+    const size_t reserve = 2048;   // static LDS of the kernel
+    const size_t avail = lds_per_block();
+    if (avail <= rec_bytes + reserve) return 0;
+    const size_t per_cell = (size_t)spb * lanes * pairs * nc * 16;
+    uint32_t wn = (uint32_t)std::sqrt((double)((avail - rec_bytes - reserve) / per_cell));
+    while (wn >= 2 && (size_t)2 * nc * wn * m_max > lanes) --wn;
+    return wn >= 2 ? wn : 0;
+}
+
 template <int MODE, int PATH, int PAIRS, bool BATCH>
-static hipError_t launch_t(const WArgs &w, uint32_t block, hipStream_t st) {
+static hipError_t launch_t(const WArgs &w0, uint32_t block, hipStream_t st) {
     typedef typename WGridder<MODE, PATH, PAIRS, BATCH>::R_t R_t;
+    WArgs w = w0;
     const uint64_t n_list = w.seg_bl ? (uint64_t)1 << 40 : w.n_list;
     uint64_t blocks = (n_list + w.streams_per_block - 1) / w.streams_per_block;
     if (w.seg_bl && !w.queue) return hipErrorInvalidValue;
     if (!blocks) return hipSuccess;
-    const size_t smem =
-        (size_t)w.streams_per_block * std::min<uint32_t>(REC, w.lanes_per_stream) * sizeof(R_t);
+    const size_t rec_bytes =
+        ((size_t)w.streams_per_block * std::min<uint32_t>(REC, w.lanes_per_stream) * sizeof(R_t) +
+         15) & ~(size_t)15;
+    const int nc = ModeTraits<MODE>::NCORR;
+    w.win = window_cells(block, w.lanes_per_stream, w.streams_per_block, PAIRS, nc, w.M,
+                         rec_bytes);
+    const size_t smem = rec_bytes + (size_t)w.streams_per_block * w.lanes_per_stream * PAIRS *
+                                        w.win * w.win * nc * 16;
     if (w.queue)
         blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, BATCH>, block, smem, blocks);
     wproj_kernel<MODE, PATH, PAIRS, BATCH><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
@@ -1203,7 +1404,12 @@ hipError_t launch_wproj_device_list(const GridArgs &a, const WListPlan &pl, void
     w.seg_bl = reinterpret_cast<const uint32_t *>(b + pl.off_seg_bl);
     w.seg_first = reinterpret_cast<const uint32_t *>(b + pl.off_seg_first);
     w.list_cls = M_hi > M ? b + pl.off_cls_sorted : nullptr;
-    return launch_wproj_class(w, mode, path, WSHAPE_DEFAULT, st);
+    // classes 19..31 walk two residue pairs per lane: two to five streams per workgroup instead
+    // of one or two (their windows are smaller, the workgroups' lanes busier; 4096^2
+    // w-projection pass 53.6 -> 50.6 ms, against 50.8 with class 17 too and 51.8 with four
+    // pairs from 25)
+    const int shape = (M_hi == M && M >= 19 && M <= 31) ? WSHAPE_PAIRS2 : WSHAPE_DEFAULT;
+    return launch_wproj_class(w, mode, path, shape, st);
 }
 
 }  // namespace bam

@@ -55,6 +55,10 @@ struct WArgs {
     // class of sorted list position i is list_cls[i], and each workgroup's stream takes its own
     // width; nullptr: every stream has width M.
     const uint8_t *list_cls;
+    // Pair kernel: on-chip window of the grid around each stream's track, win x win cells per
+    // residue pair and axis (a (win M)^2 window), set by the launch from the LDS it has room for;
+    // 0: every flush goes to the grid.
+    uint32_t win;
 };
 
 // Stream of list index li of a launch without batched facets: facet, baseline, channel block
