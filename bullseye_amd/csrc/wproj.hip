@@ -233,7 +233,8 @@ template <int MODE, int PATH> struct WPrep {
 
 // BATCH: list entries name their facet (WArgs::facet_args); without it every argument comes
 // from the kernel's own (uniform) arguments.
-template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
+// WIN: the launch gives the stream slots an on-chip window of the grid (WArgs::win).
+template <int MODE, int PATH, int PAIRS, bool BATCH, bool WIN> struct WGridder {
     static constexpr int NC = ModeTraits<MODE>::NCORR;
     static constexpr int UNR = PAIRS == 1 ? 4 : 2;   // records whose tap loads issue together
     typedef typename PathTraits<PATH>::acc_t T;
@@ -270,7 +271,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
 #pragma unroll
         for (int k = 0; k < NC; ++k) nz |= (acc[k][0].s != T(0)) | (acc[k][1].s != T(0));
         if (nz) {
-            cpx<double> *sl = wn.slot(p, cgi, x, y);
+            cpx<double> *sl = WIN ? wn.slot(p, cgi, x, y) : nullptr;
             if (sl) {
 #pragma unroll
                 for (int k = 0; k < NC; ++k) {
@@ -296,30 +297,28 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         }
     }
 
-    // Adds the window's nonzero values into the grid: lane sl of the stream takes one double
-    // (correlation, re/im) of one cell per window row, R rows per pass, so an instruction adds
-    // whole runs of a grid row (one request per 64-byte segment).
+    // Adds the nonzero values of the window's sub-box [rx0, rx0 + SX) x [ry0, ry0 + SY) (window
+    // coordinates) into the grid: lane sl of the stream takes one double (correlation, re/im) of
+    // one cell per row, R rows per pass, so an instruction adds whole runs of a grid row (one
+    // request per 64-byte segment).
     static __device__ __forceinline__ void flush_window(const GridArgs &a, uint32_t facet,
                                                         const Win &wn, const cpx<double> *wslot,
-                                                        int M, int sl) {
-        const int D = 2 * NC * wn.X;   // doubles per window row (the launch keeps D <= L)
-        const int R = wn.L / D;        // window rows per pass
+                                                        int M, int sl, int rx0, int ry0, int SX,
+                                                        int SY) {
+        const int D = 2 * NC * SX;     // doubles per row (SX <= X and the launch keeps 2 NC X <= L)
+        const int R = wn.L / D;        // rows per pass
         const int r = sl / D;
         if (r >= R) return;
         const int e = sl - r * D;
-        const int cx = e / (2 * NC), kc = e - cx * (2 * NC);
+        const int cx = rx0 + e / (2 * NC), kc = e - (e / (2 * NC)) * (2 * NC);
         const int k = kc >> 1, comp = kc & 1;
         const int x = wn.x0 + cx;
         const int jx = cx / M;
         const int am = ((x % M) + M) % M;           // u residue of the column
         const int ym = ((wn.y0 % M) + M) % M;       // v residue of the window's first row
         const double *ws = reinterpret_cast<const double *>(wslot);
-        int jy = 0, rm = r;                         // window row ry = jy * M + rm
-        while (rm >= M) {
-            rm -= M;
-            ++jy;
-        }
-        for (int ry = r; ry < wn.Y; ry += R) {
+        int jy = (ry0 + r) / M, rm = (ry0 + r) - jy * M;   // window row ry = jy * M + rm
+        for (int ry = ry0 + r; ry < ry0 + SY; ry += R) {
             int b = ym + rm;
             if (b >= M) b -= M;
             int pair = b * M + am, p = 0;           // owner: residue pair b * M + am
@@ -443,7 +442,7 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         Win wn;
         wn.lane = nullptr;
         wn.L = L;
-        wn.win = (int)w.win;
+        wn.win = WIN ? (int)w.win : 0;
         wn.X = wn.Y = wn.win * M;
         wn.x0 = wn.y0 = 0;
         wn.cgi = -1;
@@ -474,6 +473,9 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         const int kbase = BATCH ? (int)(fidx * w.g.cube) : (int)(facet * a.cube);
 
         __shared__ int s_box[16][5];   // per wavefront: its records' footprint box and slice
+        // box of the footprints of every record this lane prepared (the window flush scans only
+        // the part of the window inside the stream's box)
+        int tlx = 0x7fffffff, thx = -0x7fffffff, tly = 0x7fffffff, thy = -0x7fffffff;
         for (uint32_t st = 0; st < steps; ++st) {
             const uint32_t step0 = st * nrec;
             int bx = 0, by = 0, bc = -1;    // this lane's record: footprint origin and slice
@@ -508,6 +510,12 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
                 bx = rec.x0;
                 by = rec.y0;
                 bc = rec.cgi;
+                if (bc >= 0) {
+                    tlx = min(tlx, bx);
+                    thx = max(thx, bx);
+                    tly = min(tly, by);
+                    thy = max(thy, by);
+                }
             }
             if (wn.win && st == 0) {
                 // box of the first step's footprints (the stream's first samples in walk order),
@@ -608,8 +616,39 @@ template <int MODE, int PATH, int PAIRS, bool BATCH> struct WGridder {
         for (int p = 0; p < PAIRS; ++p)
             if (own[p] && pc[p] >= 0) flush(a, facet, pc[p], px[p], py[p], acc[p], wn, p);
         if (wn.win) {
-            __syncthreads();   // every lane's sums are in the window
-            if (live && wn.lane) flush_window(a, facet, wn, wslot, M, sl);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                tlx = min(tlx, __shfl_xor(tlx, o));
+                thx = max(thx, __shfl_xor(thx, o));
+                tly = min(tly, __shfl_xor(tly, o));
+                thy = max(thy, __shfl_xor(thy, o));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                int *b = s_box[threadIdx.x >> 6];
+                b[0] = tlx;
+                b[1] = thx;
+                b[2] = tly;
+                b[3] = thy;
+            }
+            __syncthreads();   // every lane's sums are in the window, the boxes in s_box
+            if (live && wn.lane) {
+                int lx = 0x7fffffff, hx = -0x7fffffff, ly = 0x7fffffff, hy = -0x7fffffff;
+                const int w0 = slot * (L >> 6);
+                for (int i = 0; i < (int)(nrec >> 6); ++i) {
+                    const int *b = s_box[w0 + i];
+                    lx = min(lx, b[0]);
+                    hx = max(hx, b[1]);
+                    ly = min(ly, b[2]);
+                    hy = max(hy, b[3]);
+                }
+                // cells flushed into the window lie in the footprints' box, [lx, hx + M) x
+                // [ly, hy + M); scan its part inside the window
+                const int sx0 = max(wn.x0, lx), sx1 = min(wn.x0 + wn.X, hx + M);
+                const int sy0 = max(wn.y0, ly), sy1 = min(wn.y0 + wn.Y, hy + M);
+                if (sx1 > sx0 && sy1 > sy0)
+                    flush_window(a, facet, wn, wslot, M, sl, sx0 - wn.x0, sy0 - wn.y0,
+                                 sx1 - sx0, sy1 - sy0);
+            }
         }
         // every thread of the block gets here; sums go to the launch's slots (batched facets
         // share one normalisation array, keyed by global facet)
@@ -1050,10 +1089,10 @@ static hipError_t launch_col_m(const WArgs &w, hipStream_t st) {
     return hipErrorInvalidValue;
 }
 
-template <int MODE, int PATH, int PAIRS, bool BATCH>
+template <int MODE, int PATH, int PAIRS, bool BATCH, bool WIN>
 __global__ __launch_bounds__(1024) void wproj_kernel(WArgs w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    WGridder<MODE, PATH, PAIRS, BATCH>::run(w, smem);
+    WGridder<MODE, PATH, PAIRS, BATCH, WIN>::run(w, smem);
 }
 
 // LDS a workgroup may allocate on this device.
@@ -1088,7 +1127,7 @@ static uint32_t window_cells(uint32_t block, uint32_t lanes, uint32_t spb, int p
 
 template <int MODE, int PATH, int PAIRS, bool BATCH>
 static hipError_t launch_t(const WArgs &w0, uint32_t block, hipStream_t st) {
-    typedef typename WGridder<MODE, PATH, PAIRS, BATCH>::R_t R_t;
+    typedef typename WGridder<MODE, PATH, PAIRS, BATCH, false>::R_t R_t;
     WArgs w = w0;
     const uint64_t n_list = w.seg_bl ? (uint64_t)1 << 40 : w.n_list;
     uint64_t blocks = (n_list + w.streams_per_block - 1) / w.streams_per_block;
@@ -1098,13 +1137,33 @@ static hipError_t launch_t(const WArgs &w0, uint32_t block, hipStream_t st) {
         ((size_t)w.streams_per_block * std::min<uint32_t>(REC, w.lanes_per_stream) * sizeof(R_t) +
          15) & ~(size_t)15;
     const int nc = ModeTraits<MODE>::NCORR;
-    w.win = window_cells(block, w.lanes_per_stream, w.streams_per_block, PAIRS, nc, w.M,
-                         rec_bytes);
+    // Windows pay where a stream's track crosses many cells.  Measured: a 4096^2 grid gains
+    // (w-projection pass 55.8 -> 50.6 ms), while the 128^2-2048^2 facets of the batched workload
+    // lose (200-facet batch 372 -> 354 Gtaps/s, one launch per facet 344 -> 316, though the pair
+    // kernels' requests fell 3.3x): their tracks are short and the window's per-stream clear and
+    // flush outweigh the requests saved.  So batched launches and grids under 4096^2 cells flush
+    // straight to the grid, in kernels built without the window code (its registers would
+    // spill there for nothing).
+    const bool big = (uint64_t)w.g.nx * w.g.ny >= ((uint64_t)1 << 24);
+    w.win = (!BATCH && big) ? window_cells(block, w.lanes_per_stream, w.streams_per_block, PAIRS,
+                                           nc, w.M, rec_bytes)
+                            : 0u;
     const size_t smem = rec_bytes + (size_t)w.streams_per_block * w.lanes_per_stream * PAIRS *
                                         w.win * w.win * nc * 16;
+    if constexpr (!BATCH) {
+        if (w.win) {
+            if (w.queue)
+                blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, false, true>, block,
+                                           smem, blocks);
+            wproj_kernel<MODE, PATH, PAIRS, false, true>
+                <<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
+            return hipGetLastError();
+        }
+    }
     if (w.queue)
-        blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, BATCH>, block, smem, blocks);
-    wproj_kernel<MODE, PATH, PAIRS, BATCH><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
+        blocks = persistent_blocks(wproj_kernel<MODE, PATH, PAIRS, BATCH, false>, block, smem,
+                                   blocks);
+    wproj_kernel<MODE, PATH, PAIRS, BATCH, false><<<dim3((uint32_t)blocks), dim3(block), smem, st>>>(w);
     return hipGetLastError();
 }
 

@@ -101,3 +101,20 @@ def test_wproj_walk_orders(mode, precision, path, walk):
     assert rms <= tol[0] and peak <= tol[1], (rms, peak)
     if mode not in MODES_PSF:
         np.testing.assert_allclose(norm, want_norm, rtol=1e-9)
+
+
+# Grids of at least 4096^2 cells take the residue-pair kernels' on-chip windows (the window of
+# each stream is flushed to the grid once at its end; flushes outside it go straight to the
+# grid): the same oracle bars on a 4096^2 grid, where the 20" cells make the footprints move
+# across many cells (in and out of the windows).
+@pytest.mark.parametrize("precision,path", CASES)
+@pytest.mark.parametrize("mode", ["grid_single_pol", "facet_4_cor_corrections"])
+def test_wproj_windows(mode, precision, path):
+    print("case wproj windows", mode, precision, path, flush=True)
+    (rms, peak), norm, want_norm, want = run_case(precision, path, mode, WIDE, S=23, npix=4096)
+    print("rms %.3e peak %.3e" % (rms, peak))
+    assert np.abs(want).max() > 0
+    tol = (1e-11, 1e-9) if (precision == "double" and path == PATH_FP64) else (1e-6, 2e-5)
+    assert rms <= tol[0] and peak <= tol[1], (rms, peak)
+    if mode not in MODES_PSF:
+        np.testing.assert_allclose(norm, want_norm, rtol=1e-9)
